@@ -1,0 +1,158 @@
+"""Benchmark: fused colour-gradient step on one MI355X.
+
+Workload (the metric's configuration): D3Q19 periodic 128^3 domain holding a diffuse drop
+(R = 24, tanh profile of width 4, blue inside), tau = 0.8, sigma = 0.01, beta = 0.7, FP32 moment
+storage, inputs resident in HBM. A step is one fused update of all 2,097,152 cells. The D3Q27
+128^3 figure of the same workload is reported alongside.
+
+value = MLUPS = cells * steps / seconds / 1e6 over exactly --steps timed steps (HIP events on
+the domain's stream, bracketed by device synchronisation). roofline.achieved = 88 B per cell
+update (11 FP32 fields read + 11 written) * cells / average step duration. cpu_baseline = the
+reference's own update (compiled for the host from the reference sources, oracle/_ref) on a
+bounded sample of the same workload, on this host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--kernel auto|gather|march]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE_F32 = 88
+
+
+def _load():
+    import __graft_entry__ as ge
+    return ge._import_package()
+
+
+def measure(cg, q: int, n: int, steps: int, warmup: int, kernel: int):
+    cfg = cg.Config(nx=n, ny=n, nz=n, q=q, tau=0.8, sigma=0.01, beta=0.7)
+    with cg.Domain(cfg, kernel=kernel) as dom:
+        dom.init_bubble(radius=24.0 * n / 128.0, width=4.0)
+        m0 = dom.reduce()
+        dom.step(warmup)
+        dom.sync()
+        ms = dom.time_steps(steps)
+        m1 = dom.reduce()
+    cells = cfg.interior_cells
+    sec = ms / 1e3
+    mlups = cells * steps / sec / 1e6
+    achieved = BYTES_PER_UPDATE_F32 * cells / (sec / steps) / 1e9
+    return dict(
+        mlups=mlups,
+        ms_per_step=ms / steps,
+        achieved_gbs=achieved,
+        mass_drift_rel=[float((m1[0] - m0[0]) / m0[0]), float((m1[1] - m0[1]) / m0[1])],
+        bad_cells=int(m1[5]),
+    )
+
+
+def traffic_from_profiles(workload: str):
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(seconds_budget: float = 20.0):
+    """The reference's own update (host build from the reference sources, periodic-y bubble,
+    128^3 interior) timed on this host; falls back to the oracle port if that build is absent."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_bubble128_q19")
+    if os.path.exists(exe):
+        probe = subprocess.run([exe, "time_bubble_tanh", "4", "1", "-"], env=env, capture_output=True, text=True,
+                               check=True, timeout=300)
+        sps = float(probe.stdout.split()[1])
+        k = max(2, min(200, int(seconds_budget / max(sps, 1e-3))))
+        run = subprocess.run([exe, "time_bubble_tanh", "4", str(k), "-"], env=env, capture_output=True, text=True,
+                             check=True, timeout=600)
+        sps = float(run.stdout.split()[1])
+        return {"value": 2097152 / sps / 1e6, "unit": "MLUPS", "cores": threads, "kind": "reference",
+                "sample": f"{k} timed steps (after 1 warm-up) of the same D3Q19 128^3 drop; reference lbm.cuh RCS "
+                          f"built for the host, y wrapped through its own wrapy"}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import cgref
+    p = cgref.Params(nx=128, ny=128, nz=128, q=19, tau=0.8, sigma=0.01, beta=0.7)
+    A = cgref.init_bubble(p, radius=24.0, width=4.0)
+    B = A.copy()
+    cgref.step(p, A, B, 1)
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < seconds_budget or k < 2:
+        cgref.step(p, B, A, k + 2)
+        A, B = B, A
+        k += 1
+    dt = (time.perf_counter() - t0) / k
+    return {"value": 2097152 / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{k} steps of the same D3Q19 128^3 drop with the oracle"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "gather", "march"])
+    ap.add_argument("--n", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the update of one domain "
+              f"runs on one GPU; multi-GPU sharding is a design note, not a built path)", file=sys.stderr)
+        return 2
+    cg = _load()
+    kern = {"auto": cg.KERNEL_AUTO, "gather": cg.KERNEL_GATHER, "march": cg.KERNEL_MARCH}[args.kernel]
+    print(f"bench: D3Q19 {args.n}^3 warmup {args.warmup} steps {args.steps}", file=sys.stderr, flush=True)
+    r19 = measure(cg, 19, args.n, args.steps, args.warmup, kern)
+    print(f"bench: D3Q19 {r19['mlups']:.0f} MLUPS; D3Q27 next", file=sys.stderr, flush=True)
+    r27 = measure(cg, 27, args.n, args.steps, args.warmup, kern)
+    print(f"bench: D3Q27 {r27['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    workload = f"bubble{args.n}_d3q19"
+    traffic = traffic_from_profiles(workload)
+    cpu = None
+    if not args.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline()
+    line = {
+        "metric": "MLUPS (D3Q19 128^3 fused colour-gradient step, % of HBM roofline)",
+        "value": round(r19["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r19["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: periodic 128^3 domain, diffuse drop R=24 (tanh, w=4), rho=1, u=Pi=0 at t=0",
+        "config": {"workload": workload, "grid": [args.n] * 3, "q": 19, "tau": 0.8, "sigma": 0.01, "beta": 0.7,
+                   "drop_radius": 24.0 * args.n / 128.0, "storage": "f32", "kernel": args.kernel},
+        "roofline": {"bound": "hbm", "achieved": round(r19["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r19["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+        "d3q27": {"value": round(r27["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(r27["ms_per_step"], 5),
+                  "roofline": {"bound": "hbm", "achieved": round(r27["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(r27["achieved_gbs"] / HBM_PEAK_GBS, 4)}},
+        "mass_drift_rel": {"d3q19": r19["mass_drift_rel"], "d3q27": r27["mass_drift_rel"]},
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,158 @@
+/* cglbm — MI355X (gfx950) colour-gradient lattice Boltzmann update behind a plain C ABI.
+ *
+ * The reference has no FFI: its solver update sits behind inline host launchers that take
+ * non-owning views of eleven device arrays (launch.cuh:14-94, kernels.cuh:10-30), with all
+ * configuration compile-time (constants.cuh, -DLBM_D3Q19). This header replaces that boundary
+ * one to one, with the configuration moved to a runtime struct:
+ *
+ *   MomentsDevice (memory.cuh:14-29) ..................... cglbm_moments
+ *   constants.cuh:9-93 + -DLBM_D3Q19 (compile.sh:9-26) ... cglbm_config
+ *   launch_RCS (launch.cuh:89-94) ........................ cglbm_launch_rcs
+ *   launch_JetBoundaryConditions (launch.cuh:78-83) ...... cglbm_launch_ybc
+ *   launch_InitStaticBubble/InitRTI/InitJet (launch.cuh:14-72) ... cglbm_init_*
+ *   allocate_device_memory / free_device_memory (memory.cuh:153-197) ... cglbm_domain_create/destroy
+ *   swap_moments (memory.cuh:200-203) .................... inside cglbm_step
+ *   copy_out_D2H (memory.cuh:209-217) .................... cglbm_copy_out
+ *   write_vti_step_device (io/vtiWriter.cuh:151-164) ..... cglbm_write_vti
+ *   MlupsCounter (utilities/mlups_count.cuh:14-144) ...... cglbm_time_steps
+ *
+ * Every entry point returns 0 on success or a negative CGLBM_E* code (no exit(), unlike
+ * CUDA_CHECK in utilities/cudaUtilities.cuh:9-19); cglbm_last_error() gives the message of the
+ * calling thread's last failure. Streams are hipStream_t passed as void* (NULL = the null
+ * stream). Step launches are asynchronous on their stream; a domain is used from one host
+ * thread at a time.
+ *
+ * Geometry. nx, ny, nz count interior cells (the reference's NX-2, NY-2, NZ-2). x and z are
+ * periodic (modular wrap; the reference's ghost columns, which are never read or written by the
+ * update, are not stored). y is either periodic, or bounded by the reference's two ghost planes
+ * (stored y = 0 and y = ny + 1) carrying the jet inlet/outlet (jet/boundary_conditions.cuh) or
+ * the RTI wall copies (rti/rtiIn.cuh:77-119). Arithmetic is IEEE binary32 with the reference's
+ * operation order, so results match the reference's update bit for bit.
+ */
+#ifndef CGLBM_H
+#define CGLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGLBM_ABI_VERSION 1
+
+enum {
+    CGLBM_OK = 0,
+    CGLBM_EINVAL = -1,   /* invalid argument or configuration */
+    CGLBM_ENOMEM = -2,   /* device or host allocation failed / over the memory budget */
+    CGLBM_EHIP = -3,     /* HIP runtime error */
+    CGLBM_EIO = -4,      /* file output failed */
+    CGLBM_ESTATE = -5,   /* call not valid in the object's current state */
+};
+
+enum { CGLBM_Y_PERIODIC = 0, CGLBM_Y_JET = 1, CGLBM_Y_WALLS = 2 };
+enum { CGLBM_OUTLET_CONVECTIVE = 0, CGLBM_OUTLET_ZERO_GRADIENT = 1 };
+enum { CGLBM_STORE_F32 = 0, CGLBM_STORE_F16 = 1 };
+enum { CGLBM_KERNEL_AUTO = 0, CGLBM_KERNEL_GATHER = 1, CGLBM_KERNEL_MARCH = 2 };
+
+/* Eleven device arrays, same names and order as MomentsDevice (memory.cuh:14-29). */
+typedef struct cglbm_moments {
+    float *rhor, *rhob;
+    float *ux, *uy, *uz;
+    float *Pixx, *Pixy, *Piyy, *Piyz, *Pizz, *Pixz;
+} cglbm_moments;
+
+/* Runtime form of constants.cuh:9-93 and the stencil macro. */
+typedef struct cglbm_config {
+    int32_t nx, ny, nz;       /* interior cells */
+    int32_t q;                /* 19 or 27 (stencil_ct.cuh:419-425) */
+    int32_t ybc;              /* CGLBM_Y_* */
+    int32_t outlet;           /* CGLBM_OUTLET_* (jet only; convective is the reference's) */
+    float tau;                /* taub: one relaxation time for both phases (constants.cuh:82) */
+    float sigma;              /* surface tension (constants.cuh:90) */
+    float beta;               /* recolouring beta (constants.cuh:91) */
+    float jet_x0, jet_z0;     /* inlet disk centre, interior coordinates ((n-1)/2 = reference) */
+    float jet_radius;         /* constants.cuh:32 */
+    float jet_velocity;       /* constants.cuh:35 */
+    float jet_noise;          /* relative inlet velocity perturbation; 0 = reference */
+    uint32_t seed;            /* keys the inlet perturbation */
+    float gravity;            /* body-force acceleration along -y; 0 = reference (no forcing) */
+    float atwood;             /* density-weighted forcing F = atwood*(rhob - rhor)*gravity*(-y) */
+    int32_t conservative_omega2; /* 0 = reference; 1 = skip Omega2 where |F| <= 1e-4 */
+} cglbm_config;
+
+/* Where cell (x, y, z) lives inside each array: origin + x + y*pitch_y + z*pitch_z, with y the
+ * stored index (0 = inlet ghost plane when ybc != periodic). The compact layout is
+ * {0, nx, nx*nyt}; the reference's ghost-column layout of an NX*NY*NZ array is
+ * {1 + NX*NY (+ NX for a periodic y), NX, NX*NY}. */
+typedef struct cglbm_layout {
+    int64_t origin, pitch_y, pitch_z;
+} cglbm_layout;
+
+const char *cglbm_last_error(void);
+int cglbm_abi_version(void);
+/* Fills the compact layout of a configuration. */
+int cglbm_compact_layout(const cglbm_config *cfg, cglbm_layout *out);
+
+/* ---- Launch-level entry points: caller-owned arrays (launch.cuh replacement) ------------ */
+
+/* One fused update of every interior cell, A -> B (RCSKernel, kernels.cu:119-131). When
+ * ybc != periodic the ghost planes of B are written in the same launch sequence, so this one
+ * call replaces main.cu:41 and main.cu:43. `step` keys the inlet perturbation. A and B must
+ * not alias. kernel = CGLBM_KERNEL_*. */
+int cglbm_launch_rcs(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
+                     const cglbm_layout *layout, int64_t step, int32_t kernel, void *stream);
+
+/* Boundary planes only: JetBoundaryConditions(B, Aold) (kernels.cu:102-114) or the wall copies.
+ * B == Aold is allowed (main.cu:26). */
+int cglbm_launch_ybc(const cglbm_moments *B, const cglbm_moments *Aold, const cglbm_config *cfg,
+                     const cglbm_layout *layout, int64_t step, void *stream);
+
+/* ---- Domain objects: library-owned ping-pong state (LbmDevice replacement) -------------- */
+
+typedef struct cglbm_domain cglbm_domain;
+
+/* Allocates A and B (2 x 11 arrays, zeroed) plus a private stream. storage = CGLBM_STORE_*. */
+int cglbm_domain_create(const cglbm_config *cfg, int32_t storage, cglbm_domain **out);
+int cglbm_domain_destroy(cglbm_domain *dom);
+/* Current state (which = 0) or the other buffer (which = 1); FP32 storage only. */
+int cglbm_domain_moments(cglbm_domain *dom, int32_t which, cglbm_moments *out);
+int cglbm_domain_config(const cglbm_domain *dom, cglbm_config *out);
+/* Device bytes held by a domain of this configuration. */
+int64_t cglbm_domain_bytes(const cglbm_config *cfg, int32_t storage);
+int cglbm_domain_set_kernel(cglbm_domain *dom, int32_t kernel);
+int64_t cglbm_domain_step_count(const cglbm_domain *dom);
+void *cglbm_domain_stream(cglbm_domain *dom);
+
+/* Initial conditions (computed on the host in double, uploaded; synchronous). */
+int cglbm_init_bubble(cglbm_domain *dom, double x0, double y0, double z0, double radius, double width,
+                      int32_t sharp);
+int cglbm_init_jet(cglbm_domain *dom);
+int cglbm_init_rti(cglbm_domain *dom, double amplitude, double width, int32_t n_random_modes,
+                   double random_amplitude, uint32_t seed);
+
+/* nsteps of (update, boundary planes, swap) on the domain's stream (main.cu:38-55). */
+int cglbm_step(cglbm_domain *dom, int32_t nsteps);
+/* Same, timed with HIP events recorded on the domain's stream around the launches;
+ * ms_total receives the elapsed time (MlupsCounter, mlups_count.cuh:41-144). */
+int cglbm_time_steps(cglbm_domain *dom, int32_t nsteps, double *ms_total);
+int cglbm_sync(cglbm_domain *dom);
+
+/* Host <-> device copies of the current state, compact layout, fields in cglbm_moments order,
+ * each nx*nyt*nz floats (nyt = ny, or ny + 2 with ghost planes). field_mask bit f selects field
+ * f; the selected fields are packed consecutively in `host`. Synchronous. */
+int cglbm_copy_out(cglbm_domain *dom, uint32_t field_mask, float *host);
+int cglbm_copy_in(cglbm_domain *dom, uint32_t field_mask, const float *host);
+
+/* FP64 diagnostics of the current state over interior cells, fixed summation order:
+ * out[0] = sum rhor, out[1] = sum rhob, out[2..4] = sum rho*u, out[5] = count of non-finite or
+ * negative-density cells. Synchronous. */
+int cglbm_reduce(cglbm_domain *dom, double out[6]);
+
+/* ASCII VTI with rhor rhob ux uy uz over the stored extent (io/vtiWriter.cuh:60-145). */
+int cglbm_write_vti(cglbm_domain *dom, const char *path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

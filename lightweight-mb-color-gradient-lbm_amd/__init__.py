@@ -1,0 +1,285 @@
+"""cglbm — MI355X colour-gradient lattice Boltzmann update (Python side of the C ABI).
+
+Thin ctypes binding of ``libcglbm.so`` (built in this directory from ``csrc/``; declarations in
+``include/cglbm.h``). It mirrors the reference's host interface for the solver update:
+
+* :class:`Moments`  — the eleven device arrays of ``MomentsDevice`` (memory.cuh:14-29);
+* :func:`launch_rcs` / :func:`launch_ybc` — ``launch_RCS`` / ``launch_JetBoundaryConditions``
+  (launch.cuh:78-94);
+* :class:`Domain`   — ``LbmDevice`` ownership plus the ``main.cu:38-55`` step loop, the case
+  initialisers, ``copy_out_D2H``, ``write_vti_step_device`` and the MLUPS timer.
+
+Errors raise :class:`CglbmError` carrying the library's message. There is no CPU fallback: if
+the library is missing or has no GPU, every call fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field, fields
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libcglbm.so")
+FIELDS = ("rhor", "rhob", "ux", "uy", "uz", "Pixx", "Pixy", "Piyy", "Piyz", "Pizz", "Pixz")
+
+Y_PERIODIC, Y_JET, Y_WALLS = 0, 1, 2
+OUTLET_CONVECTIVE, OUTLET_ZERO_GRADIENT = 0, 1
+STORE_F32, STORE_F16 = 0, 1
+KERNEL_AUTO, KERNEL_GATHER, KERNEL_MARCH = 0, 1, 2
+
+# Every symbol include/cglbm.h declares (checked by the CPU test suite).
+EXPORTS = (
+    "cglbm_last_error", "cglbm_abi_version", "cglbm_compact_layout", "cglbm_launch_rcs", "cglbm_launch_ybc",
+    "cglbm_domain_create", "cglbm_domain_destroy", "cglbm_domain_moments", "cglbm_domain_config",
+    "cglbm_domain_bytes", "cglbm_domain_set_kernel", "cglbm_domain_step_count", "cglbm_domain_stream",
+    "cglbm_init_bubble", "cglbm_init_jet", "cglbm_init_rti", "cglbm_step", "cglbm_time_steps", "cglbm_sync",
+    "cglbm_copy_out", "cglbm_copy_in", "cglbm_reduce", "cglbm_write_vti",
+)
+
+
+class CglbmError(RuntimeError):
+    pass
+
+
+class Moments(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in FIELDS]
+
+
+class Layout(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_int64), ("pitch_y", ctypes.c_int64), ("pitch_z", ctypes.c_int64)]
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("q", ctypes.c_int32), ("ybc", ctypes.c_int32), ("outlet", ctypes.c_int32),
+        ("tau", ctypes.c_float), ("sigma", ctypes.c_float), ("beta", ctypes.c_float),
+        ("jet_x0", ctypes.c_float), ("jet_z0", ctypes.c_float),
+        ("jet_radius", ctypes.c_float), ("jet_velocity", ctypes.c_float),
+        ("jet_noise", ctypes.c_float), ("seed", ctypes.c_uint32),
+        ("gravity", ctypes.c_float), ("atwood", ctypes.c_float),
+        ("conservative_omega2", ctypes.c_int32),
+    ]
+
+
+@dataclass
+class Config:
+    """Runtime form of constants.cuh:9-93 (field meanings in include/cglbm.h)."""
+    nx: int
+    ny: int
+    nz: int
+    q: int = 19
+    ybc: int = Y_PERIODIC
+    outlet: int = OUTLET_CONVECTIVE
+    tau: float = 0.8
+    sigma: float = 0.01
+    beta: float = 0.7
+    jet_x0: float = field(default=None)  # type: ignore[assignment]
+    jet_z0: float = field(default=None)  # type: ignore[assignment]
+    jet_radius: float = 8.0
+    jet_velocity: float = 0.05
+    jet_noise: float = 0.0
+    seed: int = 0
+    gravity: float = 0.0
+    atwood: float = 0.0
+    conservative_omega2: int = 0
+
+    def __post_init__(self):
+        if self.jet_x0 is None:
+            self.jet_x0 = (self.nx - 1) / 2.0
+        if self.jet_z0 is None:
+            self.jet_z0 = (self.nz - 1) / 2.0
+
+    @property
+    def nyt(self) -> int:
+        return self.ny if self.ybc == Y_PERIODIC else self.ny + 2
+
+    @property
+    def shape(self):
+        return (11, self.nz, self.nyt, self.nx)
+
+    @property
+    def interior_cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def c(self) -> _Config:
+        return _Config(*[getattr(self, f.name) for f in fields(self)])
+
+
+_lib = None
+
+
+def lib():
+    """Load libcglbm.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CglbmError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() or `make -C` the package")
+    L = ctypes.CDLL(LIB_PATH)
+    C = ctypes.POINTER(_Config)
+    M = ctypes.POINTER(Moments)
+    LP = ctypes.POINTER(Layout)
+    D = ctypes.c_void_p
+    sig = {
+        "cglbm_last_error": ([], ctypes.c_char_p),
+        "cglbm_abi_version": ([], ctypes.c_int),
+        "cglbm_compact_layout": ([C, LP], ctypes.c_int),
+        "cglbm_launch_rcs": ([M, M, C, LP, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p], ctypes.c_int),
+        "cglbm_launch_ybc": ([M, M, C, LP, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
+        "cglbm_domain_create": ([C, ctypes.c_int32, ctypes.POINTER(D)], ctypes.c_int),
+        "cglbm_domain_destroy": ([D], ctypes.c_int),
+        "cglbm_domain_moments": ([D, ctypes.c_int32, M], ctypes.c_int),
+        "cglbm_domain_config": ([D, C], ctypes.c_int),
+        "cglbm_domain_bytes": ([C, ctypes.c_int32], ctypes.c_int64),
+        "cglbm_domain_set_kernel": ([D, ctypes.c_int32], ctypes.c_int),
+        "cglbm_domain_step_count": ([D], ctypes.c_int64),
+        "cglbm_domain_stream": ([D], ctypes.c_void_p),
+        "cglbm_init_bubble": ([D] + [ctypes.c_double] * 5 + [ctypes.c_int32], ctypes.c_int),
+        "cglbm_init_jet": ([D], ctypes.c_int),
+        "cglbm_init_rti": ([D, ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_double, ctypes.c_uint32],
+                           ctypes.c_int),
+        "cglbm_step": ([D, ctypes.c_int32], ctypes.c_int),
+        "cglbm_time_steps": ([D, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "cglbm_sync": ([D], ctypes.c_int),
+        "cglbm_copy_out": ([D, ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
+        "cglbm_copy_in": ([D, ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
+        "cglbm_reduce": ([D, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "cglbm_write_vti": ([D, ctypes.c_char_p], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().cglbm_last_error().decode(errors="replace")
+        raise CglbmError(f"{what} failed ({rc}): {msg}")
+
+
+def compact_layout(cfg: Config) -> Layout:
+    out = Layout()
+    cc = cfg.c()
+    _check(lib().cglbm_compact_layout(ctypes.byref(cc), ctypes.byref(out)), "cglbm_compact_layout")
+    return out
+
+
+def launch_rcs(A: Moments, B: Moments, cfg: Config, layout: Layout | None = None, step: int = 1,
+               kernel: int = KERNEL_AUTO, stream: int | None = None):
+    """``launch_RCS`` + the jet boundary kernel (launch.cuh:78-94) on caller-owned arrays."""
+    cc = cfg.c()
+    L = layout if layout is not None else compact_layout(cfg)
+    _check(lib().cglbm_launch_rcs(ctypes.byref(A), ctypes.byref(B), ctypes.byref(cc), ctypes.byref(L), step, kernel,
+                                  stream), "cglbm_launch_rcs")
+
+
+def launch_ybc(B: Moments, Aold: Moments, cfg: Config, layout: Layout | None = None, step: int = 0,
+               stream: int | None = None):
+    cc = cfg.c()
+    L = layout if layout is not None else compact_layout(cfg)
+    _check(lib().cglbm_launch_ybc(ctypes.byref(B), ctypes.byref(Aold), ctypes.byref(cc), ctypes.byref(L), step,
+                                  stream), "cglbm_launch_ybc")
+
+
+def domain_bytes(cfg: Config, storage: int = STORE_F32) -> int:
+    cc = cfg.c()
+    return int(lib().cglbm_domain_bytes(ctypes.byref(cc), storage))
+
+
+class Domain:
+    """A library-owned domain (ping-pong moment state on the GPU)."""
+
+    def __init__(self, cfg: Config, storage: int = STORE_F32, kernel: int = KERNEL_AUTO):
+        self.cfg = cfg
+        self.storage = storage
+        h = ctypes.c_void_p()
+        cc = cfg.c()
+        _check(lib().cglbm_domain_create(ctypes.byref(cc), storage, ctypes.byref(h)), "cglbm_domain_create")
+        self._h = h
+        if kernel != KERNEL_AUTO:
+            self.set_kernel(kernel)
+
+    # -- lifecycle ---------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().cglbm_domain_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- configuration ------------------------------------------------------------------------
+    def set_kernel(self, kernel: int):
+        _check(lib().cglbm_domain_set_kernel(self._h, kernel), "cglbm_domain_set_kernel")
+
+    @property
+    def step_count(self) -> int:
+        return int(lib().cglbm_domain_step_count(self._h))
+
+    def moments(self, which: int = 0) -> Moments:
+        m = Moments()
+        _check(lib().cglbm_domain_moments(self._h, which, ctypes.byref(m)), "cglbm_domain_moments")
+        return m
+
+    # -- initial conditions -------------------------------------------------------------------
+    def init_bubble(self, radius: float, width: float = 4.0, sharp: bool = False, center=None):
+        c = self.cfg
+        if center is None:
+            center = ((c.nx - 1) / 2.0, (c.ny - 1) / 2.0, (c.nz - 1) / 2.0)
+        _check(lib().cglbm_init_bubble(self._h, *center, radius, width, int(sharp)), "cglbm_init_bubble")
+
+    def init_jet(self):
+        _check(lib().cglbm_init_jet(self._h), "cglbm_init_jet")
+
+    def init_rti(self, amplitude=2.0, width=4.0, n_random_modes=0, random_amplitude=0.0, seed=0):
+        _check(lib().cglbm_init_rti(self._h, amplitude, width, n_random_modes, random_amplitude, seed),
+               "cglbm_init_rti")
+
+    # -- stepping -----------------------------------------------------------------------------
+    def step(self, nsteps: int = 1):
+        _check(lib().cglbm_step(self._h, nsteps), "cglbm_step")
+
+    def time_steps(self, nsteps: int) -> float:
+        ms = ctypes.c_double()
+        _check(lib().cglbm_time_steps(self._h, nsteps, ctypes.byref(ms)), "cglbm_time_steps")
+        return ms.value
+
+    def sync(self):
+        _check(lib().cglbm_sync(self._h), "cglbm_sync")
+
+    # -- data ---------------------------------------------------------------------------------
+    def get_state(self, mask: int = 0x7FF) -> np.ndarray:
+        nf = bin(mask).count("1")
+        out = np.empty((nf,) + self.cfg.shape[1:], dtype=np.float32)
+        _check(lib().cglbm_copy_out(self._h, mask, out.ctypes.data), "cglbm_copy_out")
+        return out
+
+    def set_state(self, state: np.ndarray, mask: int = 0x7FF):
+        a = np.ascontiguousarray(state, dtype=np.float32)
+        nf = bin(mask).count("1")
+        if a.shape != (nf,) + self.cfg.shape[1:]:
+            raise CglbmError(f"state shape {a.shape} != {(nf,) + self.cfg.shape[1:]}")
+        _check(lib().cglbm_copy_in(self._h, mask, a.ctypes.data), "cglbm_copy_in")
+
+    def reduce(self) -> np.ndarray:
+        out = (ctypes.c_double * 6)()
+        _check(lib().cglbm_reduce(self._h, out), "cglbm_reduce")
+        return np.array(out[:], dtype=np.float64)
+
+    def write_vti(self, path: str):
+        _check(lib().cglbm_write_vti(self._h, path.encode()), "cglbm_write_vti")

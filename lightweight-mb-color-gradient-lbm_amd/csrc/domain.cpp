@@ -1,0 +1,383 @@
+// Library-owned domains: the LbmDevice/LbmHost ownership of memory.cuh:14-219, the step loop
+// of main.cu:38-55, the case initialisers of bubble/bubbleIn.cuh, jet/jetIn.cuh and
+// rti/rtiIn.cuh, the D2H copy of memory.cuh:209-217, the VTI writer of io/vtiWriter.cuh and
+// the event timing of utilities/mlups_count.cuh.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "cglbm.h"
+#include "kernels.hpp"
+#include "lattice.hpp"
+#include "state.hpp"
+
+using namespace cglbm;
+
+struct cglbm_domain {
+    cglbm_config cfg{};
+    int storage = CGLBM_STORE_F32;
+    int kernel = CGLBM_KERNEL_AUTO;
+    int device = 0;
+    int64_t fstride = 0;
+    float *buf[2] = {nullptr, nullptr};
+    cglbm_moments view[2]{};
+    int cur = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double *d_partials = nullptr;
+    double *d_result = nullptr;
+    int64_t steps = 0;
+    cglbm_layout layout{};
+};
+
+namespace {
+
+int herr(hipError_t e, const char *what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return set_error(CGLBM_EHIP, m.c_str());
+}
+
+#define HTRY(expr)                                  \
+    do {                                            \
+        hipError_t e_ = (expr);                     \
+        if (e_ != hipSuccess) return herr(e_, #expr); \
+    } while (0)
+
+void make_views(cglbm_domain *d) {
+    for (int b = 0; b < 2; ++b) {
+        float **v = reinterpret_cast<float **>(&d->view[b]);
+        for (int f = 0; f < 11; ++f) v[f] = d->buf[b] + f * d->fstride;
+    }
+}
+
+void release(cglbm_domain *d) {
+    if (!d) return;
+    for (auto &b : d->buf)
+        if (b) (void)hipFree(b);
+    if (d->d_partials) (void)hipFree(d->d_partials);
+    if (d->d_result) (void)hipFree(d->d_result);
+    if (d->ev0) (void)hipEventDestroy(d->ev0);
+    if (d->ev1) (void)hipEventDestroy(d->ev1);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+// Host staging of the 11 current fields (compact layout).
+int upload(cglbm_domain *d, const std::vector<float> &host) {
+    const int64_t n = stored_cells(d->cfg);
+    float *const *v = reinterpret_cast<float *const *>(&d->view[d->cur]);
+    for (int f = 0; f < 11; ++f)
+        HTRY(hipMemcpyAsync(v[f], host.data() + f * n, sizeof(float) * n, hipMemcpyHostToDevice, d->stream));
+    HTRY(hipStreamSynchronize(d->stream));
+    return CGLBM_OK;
+}
+
+int apply_ybc_inplace(cglbm_domain *d) {
+    if (d->cfg.ybc == CGLBM_Y_PERIODIC) return CGLBM_OK;
+    BcArgs b = make_bc_args(&d->view[d->cur], &d->view[d->cur], &d->cfg, &d->layout, 0);
+    HTRY(launch_ybc(b, d->stream));
+    HTRY(hipStreamSynchronize(d->stream));
+    return CGLBM_OK;
+}
+
+int check_dom(const cglbm_domain *d) {
+    if (!d) return set_error(CGLBM_EINVAL, "null domain");
+    return CGLBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t cglbm_domain_bytes(const cglbm_config *cfg, int32_t storage) {
+    if (validate_config(cfg)) return -1;
+    (void)storage;
+    return 2 * 11 * field_stride(*cfg) * static_cast<int64_t>(sizeof(float)) +
+           static_cast<int64_t>(sizeof(double)) * (kReduceBlocks * 6 + 8);
+}
+
+// allocate_device_memory (memory.cuh:153-164): two zeroed 11-field states.
+int cglbm_domain_create(const cglbm_config *cfg, int32_t storage, cglbm_domain **out) {
+    if (!out) return set_error(CGLBM_EINVAL, "null output");
+    *out = nullptr;
+    if (int rc = validate_config(cfg)) return rc;
+    if (storage != CGLBM_STORE_F32) return set_error(CGLBM_EINVAL, "storage mode not available for single domains");
+    cglbm_domain *d = new (std::nothrow) cglbm_domain;
+    if (!d) return set_error(CGLBM_ENOMEM, "host allocation failed");
+    d->cfg = *cfg;
+    d->storage = storage;
+    d->fstride = field_stride(*cfg);
+    if (hipGetDevice(&d->device) != hipSuccess) {
+        release(d);
+        return set_error(CGLBM_EHIP, "no HIP device");
+    }
+    const size_t bytes = sizeof(float) * 11 * static_cast<size_t>(d->fstride);
+    for (int b = 0; b < 2; ++b) {
+        if (hipMalloc(&d->buf[b], bytes) != hipSuccess) {
+            release(d);
+            return set_error(CGLBM_ENOMEM, "device allocation of the moment state failed");
+        }
+    }
+    hipError_t e = hipStreamCreate(&d->stream);
+    if (e == hipSuccess) e = hipEventCreate(&d->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&d->ev1);
+    if (e == hipSuccess) e = hipMalloc(&d->d_partials, sizeof(double) * kReduceBlocks * 6);
+    if (e == hipSuccess) e = hipMalloc(&d->d_result, sizeof(double) * 8);
+    if (e == hipSuccess) e = hipMemsetAsync(d->buf[0], 0, bytes, d->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d->buf[1], 0, bytes, d->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+    if (e != hipSuccess) {
+        release(d);
+        return herr(e, "domain setup");
+    }
+    make_views(d);
+    cglbm_compact_layout(cfg, &d->layout);
+    *out = d;
+    return CGLBM_OK;
+}
+
+int cglbm_domain_destroy(cglbm_domain *dom) {
+    if (!dom) return CGLBM_OK;
+    if (dom->stream) (void)hipStreamSynchronize(dom->stream);
+    release(dom);
+    return CGLBM_OK;
+}
+
+int cglbm_domain_moments(cglbm_domain *dom, int32_t which, cglbm_moments *out) {
+    if (int rc = check_dom(dom)) return rc;
+    if (!out || (which != 0 && which != 1)) return set_error(CGLBM_EINVAL, "bad arguments");
+    *out = dom->view[which == 0 ? dom->cur : 1 - dom->cur];
+    return CGLBM_OK;
+}
+
+int cglbm_domain_config(const cglbm_domain *dom, cglbm_config *out) {
+    if (int rc = check_dom(dom)) return rc;
+    if (!out) return set_error(CGLBM_EINVAL, "null output");
+    *out = dom->cfg;
+    return CGLBM_OK;
+}
+
+int cglbm_domain_set_kernel(cglbm_domain *dom, int32_t kernel) {
+    if (int rc = check_dom(dom)) return rc;
+    if (kernel < CGLBM_KERNEL_AUTO || kernel > CGLBM_KERNEL_MARCH) return set_error(CGLBM_EINVAL, "unknown kernel");
+    dom->kernel = kernel;
+    return CGLBM_OK;
+}
+
+int64_t cglbm_domain_step_count(const cglbm_domain *dom) { return dom ? dom->steps : -1; }
+
+void *cglbm_domain_stream(cglbm_domain *dom) { return dom ? static_cast<void *>(dom->stream) : nullptr; }
+
+// launch_InitStaticBubble (launch.cuh:14-20) with bubbleIn.cuh:15-52 (sharp) or the diffuse
+// tanh profile phi = tanh(2(R - r)/w), blue inside. Centre in interior coordinates.
+int cglbm_init_bubble(cglbm_domain *dom, double x0, double y0, double z0, double radius, double width,
+                      int32_t sharp) {
+    if (int rc = check_dom(dom)) return rc;
+    if (!(radius > 0) || (!sharp && !(width > 0))) return set_error(CGLBM_EINVAL, "radius/width must be positive");
+    const cglbm_config &c = dom->cfg;
+    const int nyt = stored_ny(c);
+    const int yoff = c.ybc == CGLBM_Y_PERIODIC ? 0 : 1;
+    const int64_t n = stored_cells(c);
+    std::vector<float> h(11 * n, 0.0f);
+    for (int z = 0; z < c.nz; ++z)
+        for (int yy = 0; yy < nyt; ++yy)
+            for (int x = 0; x < c.nx; ++x) {
+                const int64_t id = x + static_cast<int64_t>(c.nx) * (yy + static_cast<int64_t>(nyt) * z);
+                const double dx = x - x0, dy = (yy - yoff) - y0, dz = z - z0;
+                if (sharp) {
+                    const float fx = static_cast<float>(dx), fy = static_cast<float>(dy), fz = static_cast<float>(dz);
+                    const float r2 = fx * fx + fy * fy + fz * fz;
+                    const float R = static_cast<float>(radius);
+                    const float inside = static_cast<float>(r2 <= R * R);
+                    h[id] = (1.0f - inside) * 1.0f;
+                    h[n + id] = inside * 1.0f;
+                } else {
+                    const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    const double phi = std::tanh(2.0 * (radius - r) / width);
+                    h[id] = static_cast<float>(0.5 * (1.0 - phi));
+                    h[n + id] = static_cast<float>(0.5 * (1.0 + phi));
+                }
+            }
+    dom->steps = 0;
+    if (int rc = upload(dom, h)) return rc;
+    return apply_ybc_inplace(dom);
+}
+
+// launch_InitJet (launch.cuh:52-72) followed by main.cu:26's boundary call with B == Aold.
+int cglbm_init_jet(cglbm_domain *dom) {
+    if (int rc = check_dom(dom)) return rc;
+    if (dom->cfg.ybc != CGLBM_Y_JET) return set_error(CGLBM_EINVAL, "jet initialisation needs ybc = CGLBM_Y_JET");
+    const int64_t n = stored_cells(dom->cfg);
+    std::vector<float> h(11 * n, 0.0f);
+    for (int64_t i = 0; i < n; ++i) h[i] = 1.0f;  // red at rest (jetIn.cuh:16-32)
+    dom->steps = 0;
+    if (int rc = upload(dom, h)) return rc;
+    return apply_ybc_inplace(dom);
+}
+
+// launch_InitRTI (launch.cuh:42-46): rtiIn.cuh:18-71 evaluated in double (plus optional seeded
+// random modes), then the wall planes (rtiIn.cuh:77-119).
+int cglbm_init_rti(cglbm_domain *dom, double amplitude, double width, int32_t n_random_modes,
+                   double random_amplitude, uint32_t seed) {
+    if (int rc = check_dom(dom)) return rc;
+    if (!(width > 0) || n_random_modes < 0 || n_random_modes > 64) return set_error(CGLBM_EINVAL, "bad RTI parameters");
+    const cglbm_config &c = dom->cfg;
+    const int nyt = stored_ny(c);
+    const bool per = c.ybc == CGLBM_Y_PERIODIC;
+    const double PI = 3.14159265358979323846;
+    const double Lx = c.nx, Lz = c.nz;
+    const double y0 = (nyt - 1) / 2.0 + (per ? 1.0 : 0.0);
+    std::vector<double> ma(n_random_modes), mb(n_random_modes), ph(n_random_modes);
+    uint64_t s = seed;
+    for (int m = 0; m < n_random_modes; ++m) {
+        s += 0x9E3779B97F4A7C15ull;
+        ma[m] = 1 + (mix64(s) % 8);
+        s += 0x9E3779B97F4A7C15ull;
+        mb[m] = 1 + (mix64(s) % 8);
+        s += 0x9E3779B97F4A7C15ull;
+        ph[m] = 2.0 * PI * (static_cast<double>(mix64(s) >> 11) * (1.0 / 9007199254740992.0));
+    }
+    const int64_t n = stored_cells(c);
+    std::vector<float> h(11 * n, 0.0f);
+    for (int z = 0; z < c.nz; ++z)
+        for (int x = 0; x < c.nx; ++x) {
+            const double xr = x + 1.0, zr = z + 1.0;
+            double yi = y0 + amplitude * std::cos(2.0 * PI * xr / Lx) * std::cos(2.0 * PI * zr / Lz);
+            for (int m = 0; m < n_random_modes; ++m)
+                yi += random_amplitude * std::cos(2.0 * PI * (ma[m] * xr / Lx + mb[m] * zr / Lz) + ph[m]);
+            for (int yy = 0; yy < nyt; ++yy) {
+                const double yr = yy + (per ? 1.0 : 0.0);
+                const double phi = std::tanh((yr - yi) / width);
+                const int64_t id = x + static_cast<int64_t>(c.nx) * (yy + static_cast<int64_t>(nyt) * z);
+                h[id] = static_cast<float>(0.5 * (1.0 - phi));
+                h[n + id] = static_cast<float>(0.5 * (1.0 + phi));
+            }
+        }
+    dom->steps = 0;
+    if (int rc = upload(dom, h)) return rc;
+    if (c.ybc == CGLBM_Y_WALLS) return apply_ybc_inplace(dom);
+    return CGLBM_OK;
+}
+
+// main.cu:38-55: update, boundary planes, swap.
+int cglbm_step(cglbm_domain *dom, int32_t nsteps) {
+    if (int rc = check_dom(dom)) return rc;
+    if (nsteps < 0) return set_error(CGLBM_EINVAL, "negative step count");
+    for (int k = 0; k < nsteps; ++k) {
+        const int nxt = 1 - dom->cur;
+        if (int rc = launch_step(&dom->view[dom->cur], &dom->view[nxt], &dom->cfg, &dom->layout, dom->steps + 1,
+                                 dom->kernel, dom->stream))
+            return rc;
+        dom->cur = nxt;
+        ++dom->steps;
+    }
+    return CGLBM_OK;
+}
+
+int cglbm_time_steps(cglbm_domain *dom, int32_t nsteps, double *ms_total) {
+    if (int rc = check_dom(dom)) return rc;
+    if (!ms_total) return set_error(CGLBM_EINVAL, "null output");
+    HTRY(hipEventRecord(dom->ev0, dom->stream));
+    if (int rc = cglbm_step(dom, nsteps)) return rc;
+    HTRY(hipEventRecord(dom->ev1, dom->stream));
+    HTRY(hipEventSynchronize(dom->ev1));
+    float ms = 0.0f;
+    HTRY(hipEventElapsedTime(&ms, dom->ev0, dom->ev1));
+    *ms_total = ms;
+    return CGLBM_OK;
+}
+
+int cglbm_sync(cglbm_domain *dom) {
+    if (int rc = check_dom(dom)) return rc;
+    HTRY(hipStreamSynchronize(dom->stream));
+    return CGLBM_OK;
+}
+
+// copy_out_D2H (memory.cuh:209-217), any subset of the 11 fields.
+int cglbm_copy_out(cglbm_domain *dom, uint32_t field_mask, float *host) {
+    if (int rc = check_dom(dom)) return rc;
+    if (!host || (field_mask & ~0x7FFu)) return set_error(CGLBM_EINVAL, "bad arguments");
+    const int64_t n = stored_cells(dom->cfg);
+    float *const *v = reinterpret_cast<float *const *>(&dom->view[dom->cur]);
+    int k = 0;
+    for (int f = 0; f < 11; ++f)
+        if (field_mask & (1u << f))
+            HTRY(hipMemcpyAsync(host + (k++) * n, v[f], sizeof(float) * n, hipMemcpyDeviceToHost, dom->stream));
+    HTRY(hipStreamSynchronize(dom->stream));
+    return CGLBM_OK;
+}
+
+int cglbm_copy_in(cglbm_domain *dom, uint32_t field_mask, const float *host) {
+    if (int rc = check_dom(dom)) return rc;
+    if (!host || (field_mask & ~0x7FFu)) return set_error(CGLBM_EINVAL, "bad arguments");
+    const int64_t n = stored_cells(dom->cfg);
+    float *const *v = reinterpret_cast<float *const *>(&dom->view[dom->cur]);
+    int k = 0;
+    for (int f = 0; f < 11; ++f)
+        if (field_mask & (1u << f))
+            HTRY(hipMemcpyAsync(v[f], host + (k++) * n, sizeof(float) * n, hipMemcpyHostToDevice, dom->stream));
+    HTRY(hipStreamSynchronize(dom->stream));
+    return CGLBM_OK;
+}
+
+int cglbm_reduce(cglbm_domain *dom, double out[6]) {
+    if (int rc = check_dom(dom)) return rc;
+    if (!out) return set_error(CGLBM_EINVAL, "null output");
+    const cglbm_config &c = dom->cfg;
+    float *const *v = reinterpret_cast<float *const *>(&dom->view[dom->cur]);
+    HTRY(launch_reduce(v, c.nx, c.ny, c.nz, c.ybc == CGLBM_Y_PERIODIC ? 0 : 1, static_cast<uint32_t>(dom->layout.pitch_y),
+                       static_cast<uint32_t>(dom->layout.pitch_z), dom->d_partials, dom->d_result, dom->stream));
+    HTRY(hipMemcpyAsync(out, dom->d_result, sizeof(double) * 6, hipMemcpyDeviceToHost, dom->stream));
+    HTRY(hipStreamSynchronize(dom->stream));
+    return CGLBM_OK;
+}
+
+// write_vti (io/vtiWriter.cuh:92-145): ASCII ImageData, Float32 point arrays rhor rhob ux uy uz,
+// x fastest, one row per (z, y), "%.8e "-style values.
+int cglbm_write_vti(cglbm_domain *dom, const char *path) {
+    if (int rc = check_dom(dom)) return rc;
+    if (!path) return set_error(CGLBM_EINVAL, "null path");
+    const cglbm_config &c = dom->cfg;
+    const int nyt = stored_ny(c);
+    const int64_t n = stored_cells(c);
+    std::vector<float> h(5 * n);
+    if (int rc = cglbm_copy_out(dom, 0x1Fu, h.data())) return rc;
+    std::ofstream out(path);
+    if (!out) return set_error(CGLBM_EIO, "cannot open VTI file for writing");
+    out << "<?xml version=\"1.0\"?>\n";
+    out << "<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
+    out << "  <ImageData WholeExtent=\"0 " << c.nx - 1 << " 0 " << nyt - 1 << " 0 " << c.nz - 1
+        << "\" Origin=\"0 0 0\" Spacing=\"1 1 1\">\n";
+    out << "    <Piece Extent=\"0 " << c.nx - 1 << " 0 " << nyt - 1 << " 0 " << c.nz - 1 << "\">\n";
+    out << "      <PointData Scalars=\"rhor\">\n";
+    static const char *names[5] = {"rhor", "rhob", "ux", "uy", "uz"};
+    out << std::scientific << std::setprecision(8);
+    for (int f = 0; f < 5; ++f) {
+        out << "        <DataArray type=\"Float32\" Name=\"" << names[f] << "\" format=\"ascii\">\n";
+        const float *p = h.data() + f * n;
+        for (int z = 0; z < c.nz; ++z)
+            for (int y = 0; y < nyt; ++y) {
+                const float *row = p + static_cast<int64_t>(c.nx) * (y + static_cast<int64_t>(nyt) * z);
+                for (int x = 0; x < c.nx; ++x) out << row[x] << " ";
+                out << "\n";
+            }
+        out << "        </DataArray>\n";
+    }
+    out << "      </PointData>\n";
+    out << "      <CellData>\n";
+    out << "      </CellData>\n";
+    out << "    </Piece>\n";
+    out << "  </ImageData>\n";
+    out << "</VTKFile>\n";
+    if (!out) return set_error(CGLBM_EIO, "error while writing the VTI file");
+    return CGLBM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// FP64 per-phase mass and momentum diagnostics (the reference has none; lbm.cuh stores the
+// moments these sum). Deterministic: every cell goes to a fixed block and lane, each lane sums
+// its cells in a fixed order in double, blocks combine with a fixed tree, and one block adds the
+// block partials in index order. The result therefore does not depend on timing or on what else
+// runs on the GPU.
+#include "kernels.hpp"
+
+namespace cglbm {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__global__ void __launch_bounds__(kThreads) reduce_partials_kernel(const float *rr, const float *rb, const float *ux,
+                                                                   const float *uy, const float *uz, int nx, int ny,
+                                                                   int nz, int y0, uint32_t py, uint32_t pz,
+                                                                   double *partials) {
+    __shared__ double sh[6][kThreads];
+    const int64_t n = static_cast<int64_t>(nx) * ny * nz;
+    const int64_t nb = static_cast<int64_t>(gridDim.x);
+    const int64_t per_block = (n + nb - 1) / nb;
+    const int64_t lo = per_block * blockIdx.x;
+    const int64_t hi = lo + per_block < n ? lo + per_block : n;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t c = lo + threadIdx.x; c < hi; c += kThreads) {
+        const int x = static_cast<int>(c % nx);
+        const int64_t r = c / nx;
+        const int y = static_cast<int>(r % ny) + y0;
+        const int z = static_cast<int>(r / ny);
+        const uint32_t id = static_cast<uint32_t>(x) + py * static_cast<uint32_t>(y) + pz * static_cast<uint32_t>(z);
+        const double a = rr[id], b = rb[id];
+        const double rho = a + b;
+        s[0] += a;
+        s[1] += b;
+        s[2] += rho * static_cast<double>(ux[id]);
+        s[3] += rho * static_cast<double>(uy[id]);
+        s[4] += rho * static_cast<double>(uz[id]);
+        const bool bad = !(a >= 0.0) || !(b >= 0.0) || !(rho > 0.0) || !isfinite(static_cast<double>(ux[id])) ||
+                         !isfinite(static_cast<double>(uy[id])) || !isfinite(static_cast<double>(uz[id]));
+        s[5] += bad ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partials[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void reduce_final_kernel(const double *partials, int nblocks, double *result) {
+    if (threadIdx.x < 6) {
+        double s = 0.0;
+        for (int b = 0; b < nblocks; ++b) s += partials[b * 6 + threadIdx.x];
+        result[threadIdx.x] = s;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_reduce(const float *const *A, int nx, int ny, int nz, int y0, uint32_t py, uint32_t pz,
+                         double *partials, double *result, hipStream_t s) {
+    reduce_partials_kernel<<<kReduceBlocks, kThreads, 0, s>>>(A[0], A[1], A[2], A[3], A[4], nx, ny, nz, y0, py, pz,
+                                                              partials);
+    reduce_final_kernel<<<1, 64, 0, s>>>(partials, kReduceBlocks, result);
+    return hipGetLastError();
+}
+
+}  // namespace cglbm

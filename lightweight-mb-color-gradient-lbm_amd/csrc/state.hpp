@@ -1,0 +1,28 @@
+// Internal helpers shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cglbm.h"
+#include "step_common.hpp"
+
+namespace cglbm {
+
+int set_error(int code, const char *msg);
+int validate_config(const cglbm_config *c);
+int check_layout(const cglbm_config *c, const cglbm_layout *L);
+StepArgs make_step_args(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *c,
+                        const cglbm_layout *L, int64_t step);
+BcArgs make_bc_args(const cglbm_moments *B, const cglbm_moments *Aold, const cglbm_config *c,
+                    const cglbm_layout *L, int64_t step);
+int launch_step(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
+                const cglbm_layout *L, int64_t step, int kernel, hipStream_t s);
+
+inline int stored_ny(const cglbm_config &c) { return c.ybc == CGLBM_Y_PERIODIC ? c.ny : c.ny + 2; }
+inline int64_t stored_cells(const cglbm_config &c) {
+    return static_cast<int64_t>(c.nx) * stored_ny(c) * c.nz;
+}
+// Field stride in floats: padded to a multiple of 64 floats (256 B) so every field starts aligned.
+inline int64_t field_stride(const cglbm_config &c) { return (stored_cells(c) + 63) / 64 * 64; }
+
+}  // namespace cglbm

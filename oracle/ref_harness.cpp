@@ -47,6 +47,7 @@ inline label_t ref_force_y_periodic(const int y) noexcept { return wrapy(static_
 #include "bubble/bubbleIn.cuh"
 #include "rti/rtiIn.cuh"
 
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -184,6 +185,28 @@ int main(int argc, char **argv) {
         if (*p == ',') ++p;
     }
     const std::string prefix = argv[4];
+
+    if (mode == "time_bubble_tanh") {
+        // CPU-baseline timing: diffuse drop, one untimed step, then `steps` timed steps;
+        // prints seconds per step (argv[3] = number of timed steps, argv[4] ignored).
+        if (!g_periodic_y) { std::fprintf(stderr, "needs the periodic-y build\n"); return 1; }
+        HostState *A = new HostState, *B = new HostState;
+        init_bubble_tanh(*A, arg);
+        rcs_all(*A, *B);
+        std::swap(A, B);
+        const int steps = dumps.empty() ? 1 : dumps[0];
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < steps; ++k) {
+            rcs_all(*A, *B);
+            std::swap(A, B);
+        }
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("seconds_per_step %.9f cells %llu\n", dt / steps,
+                    static_cast<unsigned long long>(NX - 2) * (NY - 2) * (NZ - 2));
+        delete A;
+        delete B;
+        return 0;
+    }
 
     HostState *A = new HostState, *B = new HostState;
     if (mode == "jet") {

@@ -1,0 +1,82 @@
+"""The C-ABI library loads and exports every entry point include/cglbm.h declares; argument
+checking that needs no GPU answers with error codes, never a crash."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT, load_product
+
+HEADER = os.path.join(ROOT, "include", "cglbm.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(cglbm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    cg = load_product()
+    L = cg.lib()
+    names = declared_symbols()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, f"declared but not exported: {missing}"
+    assert sorted(cg.EXPORTS) == names, "Python binding list out of sync with the header"
+
+
+def test_abi_version():
+    cg = load_product()
+    assert cg.lib().cglbm_abi_version() == 1
+
+
+@pytest.mark.parametrize("bad", [
+    dict(q=21), dict(nx=2), dict(tau=0.5), dict(tau=float("nan")), dict(sigma=-1e-3), dict(beta=1.5),
+    dict(ybc=7), dict(outlet=3), dict(jet_noise=-0.1),
+])
+def test_invalid_configs_are_rejected(bad):
+    cg = load_product()
+    kw = dict(nx=8, ny=8, nz=8)
+    kw.update(bad)
+    cfg = cg.Config(**kw)
+    with pytest.raises(cg.CglbmError) as e:
+        cg.compact_layout(cfg)
+    assert "failed (-1)" in str(e.value)
+    assert cg.domain_bytes(cfg) == -1
+
+
+def test_compact_layout_and_bytes():
+    cg = load_product()
+    cfg = cg.Config(nx=10, ny=12, nz=6, ybc=cg.Y_JET)
+    L = cg.compact_layout(cfg)
+    assert (L.origin, L.pitch_y, L.pitch_z) == (0, 10, 10 * 14)
+    n = 10 * 14 * 6
+    stride = (n + 63) // 64 * 64
+    assert cg.domain_bytes(cfg) >= 2 * 11 * stride * 4
+
+
+def test_launch_rejects_bad_layouts_and_aliasing():
+    cg = load_product()
+    cfg = cg.Config(nx=8, ny=8, nz=8)
+    A = cg.Moments(*([ctypes.c_void_p(4096).value] * 11))
+    B = cg.Moments(*([ctypes.c_void_p(8192).value] * 11))
+    with pytest.raises(cg.CglbmError, match="layout"):
+        cg.launch_rcs(A, B, cfg, cg.Layout(0, 4, 64))
+    with pytest.raises(cg.CglbmError, match="alias"):
+        cg.launch_rcs(A, A, cfg)
+    nullB = cg.Moments()
+    with pytest.raises(cg.CglbmError, match="null"):
+        cg.launch_rcs(A, nullB, cfg)
+
+
+def test_domain_create_without_gpu_fails_cleanly():
+    cg = load_product()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(cg.CglbmError):
+        cg.Domain(cg.Config(nx=8, ny=8, nz=8))
