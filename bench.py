@@ -39,6 +39,9 @@ def measure(cg, q: int, n: int, steps: int, warmup: int, kernel: int):
     with cg.Domain(cfg, kernel=kernel) as dom:
         dom.init_bubble(radius=24.0 * n / 128.0, width=4.0)
         m0 = dom.reduce()
+        shape = None
+        if kernel != cg.KERNEL_GATHER:
+            shape = dom.autotune(3)[:3]  # tile rows, chunk length, exchange buffers
         dom.step(warmup)
         dom.sync()
         ms = dom.time_steps(steps)
@@ -53,7 +56,32 @@ def measure(cg, q: int, n: int, steps: int, warmup: int, kernel: int):
         achieved_gbs=achieved,
         mass_drift_rel=[float((m1[0] - m0[0]) / m0[0]), float((m1[1] - m0[1]) / m0[1])],
         bad_cells=int(m1[5]),
+        tile=shape,
     )
+
+
+def calibrate_copy(cg, mib: int):
+    """One hipMemcpy device-to-device of `mib` MiB through the HIP runtime the library uses: a
+    kernel of known traffic (mib MiB read + mib MiB written) for scaling FETCH_SIZE/WRITE_SIZE."""
+    import ctypes
+    cg.lib()
+    path = None
+    with open("/proc/self/maps") as f:
+        for line in f:
+            if "libamdhip64" in line:
+                path = line.split()[-1]
+                break
+    hip = ctypes.CDLL(path)
+    n = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(n)) == 0
+    hip.hipMemset(src, 1, ctypes.c_size_t(n))
+    hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3)
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(f"bench: calibration copy of {mib} MiB done", file=sys.stderr, flush=True)
 
 
 def traffic_from_profiles(workload: str):
@@ -109,6 +137,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="auto", choices=["auto", "gather", "march"])
     ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the update of one domain "
@@ -121,6 +151,8 @@ def main() -> int:
     print(f"bench: D3Q19 {r19['mlups']:.0f} MLUPS; D3Q27 next", file=sys.stderr, flush=True)
     r27 = measure(cg, 27, args.n, args.steps, args.warmup, kern)
     print(f"bench: D3Q27 {r27['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    if args.calibrate_copy:
+        calibrate_copy(cg, args.calibrate_copy)
     workload = f"bubble{args.n}_d3q19"
     traffic = traffic_from_profiles(workload)
     cpu = None
@@ -141,7 +173,8 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic: periodic 128^3 domain, diffuse drop R=24 (tanh, w=4), rho=1, u=Pi=0 at t=0",
         "config": {"workload": workload, "grid": [args.n] * 3, "q": 19, "tau": 0.8, "sigma": 0.01, "beta": 0.7,
-                   "drop_radius": 24.0 * args.n / 128.0, "storage": "f32", "kernel": args.kernel},
+                   "drop_radius": 24.0 * args.n / 128.0, "storage": "f32", "kernel": args.kernel,
+                   "tile_d3q19": r19["tile"], "tile_d3q27": r27["tile"]},
         "roofline": {"bound": "hbm", "achieved": round(r19["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r19["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,

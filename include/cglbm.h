@@ -121,6 +121,15 @@ int cglbm_domain_config(const cglbm_domain *dom, cglbm_config *out);
 /* Device bytes held by a domain of this configuration. */
 int64_t cglbm_domain_bytes(const cglbm_config *cfg, int32_t storage);
 int cglbm_domain_set_kernel(cglbm_domain *dom, int32_t kernel);
+/* Tile shape of the z-marching kernel: ty tile rows (4, 8 or 16; tiles are 32 cells wide),
+ * L planes per z-chunk, nbuf LDS exchange buffers (2: one barrier per plane; 1: half the LDS,
+ * two barriers). ty = 0 returns to automatic selection (autotuned record, else default). */
+int cglbm_domain_set_tile(cglbm_domain *dom, int32_t ty, int32_t L, int32_t nbuf);
+/* Autotuner: times each candidate tile shape for steps_per_candidate updates from the current
+ * state (the state does not advance), keeps the fastest for this domain and records it for every
+ * later domain or launch of the same (q, ybc, nx, ny, nz). Outputs may be NULL. Synchronous. */
+int cglbm_domain_autotune(cglbm_domain *dom, int32_t steps_per_candidate, int32_t *ty, int32_t *L,
+                          int32_t *nbuf, double *ms_per_step);
 int64_t cglbm_domain_step_count(const cglbm_domain *dom);
 void *cglbm_domain_stream(cglbm_domain *dom);
 

@@ -33,7 +33,8 @@ KERNEL_AUTO, KERNEL_GATHER, KERNEL_MARCH = 0, 1, 2
 EXPORTS = (
     "cglbm_last_error", "cglbm_abi_version", "cglbm_compact_layout", "cglbm_launch_rcs", "cglbm_launch_ybc",
     "cglbm_domain_create", "cglbm_domain_destroy", "cglbm_domain_moments", "cglbm_domain_config",
-    "cglbm_domain_bytes", "cglbm_domain_set_kernel", "cglbm_domain_step_count", "cglbm_domain_stream",
+    "cglbm_domain_bytes", "cglbm_domain_set_kernel", "cglbm_domain_set_tile", "cglbm_domain_autotune",
+    "cglbm_domain_step_count", "cglbm_domain_stream",
     "cglbm_init_bubble", "cglbm_init_jet", "cglbm_init_rti", "cglbm_step", "cglbm_time_steps", "cglbm_sync",
     "cglbm_copy_out", "cglbm_copy_in", "cglbm_reduce", "cglbm_write_vti",
 )
@@ -135,6 +136,9 @@ def lib():
         "cglbm_domain_config": ([D, C], ctypes.c_int),
         "cglbm_domain_bytes": ([C, ctypes.c_int32], ctypes.c_int64),
         "cglbm_domain_set_kernel": ([D, ctypes.c_int32], ctypes.c_int),
+        "cglbm_domain_set_tile": ([D, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
+        "cglbm_domain_autotune": ([D, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                   ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "cglbm_domain_step_count": ([D], ctypes.c_int64),
         "cglbm_domain_stream": ([D], ctypes.c_void_p),
         "cglbm_init_bubble": ([D] + [ctypes.c_double] * 5 + [ctypes.c_int32], ctypes.c_int),
@@ -226,6 +230,16 @@ class Domain:
     # -- configuration ------------------------------------------------------------------------
     def set_kernel(self, kernel: int):
         _check(lib().cglbm_domain_set_kernel(self._h, kernel), "cglbm_domain_set_kernel")
+
+    def set_tile(self, ty: int, L: int = 0, nbuf: int = 2):
+        _check(lib().cglbm_domain_set_tile(self._h, ty, L, nbuf), "cglbm_domain_set_tile")
+
+    def autotune(self, steps_per_candidate: int = 3):
+        """Returns (tile rows, chunk length, exchange buffers, ms per step) of the fastest shape."""
+        ty, L, nb, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        _check(lib().cglbm_domain_autotune(self._h, steps_per_candidate, ctypes.byref(ty), ctypes.byref(L),
+                                           ctypes.byref(nb), ctypes.byref(ms)), "cglbm_domain_autotune")
+        return ty.value, L.value, nb.value, ms.value
 
     @property
     def step_count(self) -> int:

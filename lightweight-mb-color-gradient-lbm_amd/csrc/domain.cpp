@@ -35,6 +35,7 @@ struct cglbm_domain {
     double *d_result = nullptr;
     int64_t steps = 0;
     cglbm_layout layout{};
+    MarchShape shape{0, 0, 0};  // ty == 0: tuned record or default
 };
 
 namespace {
@@ -173,6 +174,63 @@ int cglbm_domain_set_kernel(cglbm_domain *dom, int32_t kernel) {
 
 int64_t cglbm_domain_step_count(const cglbm_domain *dom) { return dom ? dom->steps : -1; }
 
+int cglbm_domain_set_tile(cglbm_domain *dom, int32_t ty, int32_t L, int32_t nbuf) {
+    if (int rc = check_dom(dom)) return rc;
+    if (ty == 0) {
+        dom->shape = MarchShape{0, 0, 0};
+        return CGLBM_OK;
+    }
+    if ((ty != 4 && ty != 8 && ty != 16) || L < 1 || L > dom->cfg.nz || (nbuf != 1 && nbuf != 2))
+        return set_error(CGLBM_EINVAL, "tile rows must be 4, 8 or 16, 1 <= L <= nz, nbuf 1 or 2");
+    dom->shape = MarchShape{ty, L, nbuf};
+    return CGLBM_OK;
+}
+
+// Project autotuner: every (tile rows, chunk length) candidate runs a few updates from the current
+// state into the other buffer (the state itself does not advance), timed with HIP events on the
+// domain's stream; the fastest is kept for this domain and recorded for its configuration.
+int cglbm_domain_autotune(cglbm_domain *dom, int32_t steps_per_candidate, int32_t *ty_out, int32_t *L_out,
+                          int32_t *nbuf_out, double *ms_out) {
+    if (int rc = check_dom(dom)) return rc;
+    if (steps_per_candidate < 1 || steps_per_candidate > 1000) return set_error(CGLBM_EINVAL, "bad step count");
+    const int tys[3] = {4, 8, 16};
+    const int Ls[9] = {2, 4, 6, 8, 12, 16, 24, 32, 64};
+    MarchShape best{0, 0, 0};
+    double best_ms = 1e30;
+    const int nxt = 1 - dom->cur;
+    for (int nbuf = 1; nbuf <= 2; ++nbuf)
+    for (int ty : tys)
+        for (int L : Ls) {
+            if (L > dom->cfg.nz) continue;
+            const MarchShape m{ty, L, nbuf};
+            // warm-up launch, then the timed ones
+            if (int rc = launch_step(&dom->view[dom->cur], &dom->view[nxt], &dom->cfg, &dom->layout, dom->steps + 1,
+                                     CGLBM_KERNEL_MARCH, &m, dom->stream))
+                return rc;
+            HTRY(hipEventRecord(dom->ev0, dom->stream));
+            for (int k = 0; k < steps_per_candidate; ++k)
+                if (int rc = launch_step(&dom->view[dom->cur], &dom->view[nxt], &dom->cfg, &dom->layout,
+                                         dom->steps + 1, CGLBM_KERNEL_MARCH, &m, dom->stream))
+                    return rc;
+            HTRY(hipEventRecord(dom->ev1, dom->stream));
+            HTRY(hipEventSynchronize(dom->ev1));
+            float ms = 0.0f;
+            HTRY(hipEventElapsedTime(&ms, dom->ev0, dom->ev1));
+            const double per = static_cast<double>(ms) / steps_per_candidate;
+            if (per < best_ms) {
+                best_ms = per;
+                best = m;
+            }
+        }
+    dom->shape = best;
+    tuned_shape_store(dom->cfg, best);
+    if (ty_out) *ty_out = best.ty;
+    if (L_out) *L_out = best.L;
+    if (nbuf_out) *nbuf_out = best.nbuf;
+    if (ms_out) *ms_out = best_ms;
+    return CGLBM_OK;
+}
+
 void *cglbm_domain_stream(cglbm_domain *dom) { return dom ? static_cast<void *>(dom->stream) : nullptr; }
 
 // launch_InitStaticBubble (launch.cuh:14-20) with bubbleIn.cuh:15-52 (sharp) or the diffuse
@@ -273,7 +331,7 @@ int cglbm_step(cglbm_domain *dom, int32_t nsteps) {
     for (int k = 0; k < nsteps; ++k) {
         const int nxt = 1 - dom->cur;
         if (int rc = launch_step(&dom->view[dom->cur], &dom->view[nxt], &dom->cfg, &dom->layout, dom->steps + 1,
-                                 dom->kernel, dom->stream))
+                                 dom->kernel, &dom->shape, dom->stream))
             return rc;
         dom->cur = nxt;
         ++dom->steps;

@@ -12,10 +12,36 @@ hipError_t launch_gather(const StepArgs &a, int q, hipStream_t s);
 // Boundary planes of B from B and Aold (jet inlet/outlet or wall copies); no-op when periodic.
 hipError_t launch_ybc(const BcArgs &a, hipStream_t s);
 
-// z-marching step (compact layout, fused boundary planes). Returns hipErrorInvalidValue when the
-// shape is outside what the kernel supports (caller falls back to the gather path).
-hipError_t launch_march(const StepArgs &a, int q, hipStream_t s);
+// z-marching step (any layout, fused boundary planes).
+struct MarchShape {
+    int ty;    // tile rows (4, 8 or 16; the tile is 32 wide)
+    int L;     // planes per z-chunk
+    int nbuf;  // LDS exchange buffers: 2 = one barrier per plane, 1 = two barriers, half the LDS
+};
+// Arguments of the marching kernel: both states as one base pointer each plus a field stride
+// (every field at base + f * stride), read and written through buffer resources.
+struct MarchArgs {
+    const float *A0;
+    float *B0;
+    uint32_t fsA, fsB;      // field strides (elements)
+    uint32_t spanA, spanB;  // bytes addressable from A0 / B0 (buffer range)
+    int nx, ny, nz;
+    uint32_t py, pz;
+    int ybc, outlet, conservative, forced;
+    Coeffs c;
+    float jet_x0, jet_z0, jet_r, jet_u, jet_noise;
+    uint32_t seed;
+    int64_t step;
+};
+// Fills `m` when the 11 fields of each state sit at one stride (the domain layout, or caller
+// arrays carved from one allocation); false otherwise.
+bool march_args(const StepArgs &a, MarchArgs &m);
 bool march_supports(const StepArgs &a, int q);
+MarchShape march_default_shape(const StepArgs &a, int q);
+hipError_t launch_march_shape(const StepArgs &a, int q, MarchShape m, hipStream_t s);
+hipError_t launch_march_q19(const MarchArgs &a, MarchShape m, hipStream_t s);
+hipError_t launch_march_q27(const MarchArgs &a, MarchShape m, hipStream_t s);
+hipError_t launch_march(const StepArgs &a, int q, hipStream_t s);
 
 // FP64 diagnostics: partial sums per block into `partials` (capacity kReduceBlocks * 6), then the
 // fixed-order final sum into `result` (6 doubles).

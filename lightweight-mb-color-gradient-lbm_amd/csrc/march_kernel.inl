@@ -1,0 +1,490 @@
+// z-marching fused step kernel template (included by march_q19.hip and march_q27.hip).
+//
+// Same arithmetic as lbm.cuh:20-262 (see march_core.hpp), organised for the hardware:
+//
+//  * Push form: a source cell builds its post-collision populations once, opposite directions
+//    sharing their common work.
+//  * A work-group owns a 32 x TY column tile and marches along z over a chunk of planes. Its own
+//    waves hold one source/target column per lane. Populations that stay in a column (rest, +z,
+//    -z) stay in registers; the others cross to the target column through LDS, one float2 per
+//    (direction, target).
+//  * Two ring waves build the halo sources around the tile, each only for the direction pairs
+//    that can enter it: the row wave (rows -1 and TY) the pairs with c_y != 0, the column wave
+//    (columns -1, 32 and the corners) the pairs with c_x != 0. The first and last source plane
+//    of a chunk build only the pairs with c_z != 0. The ring waves also form the phase field
+//    psi = (rr - rb)/(rr + rb) once per cell, two planes ahead, into an LDS ring (halo 2).
+//  * Target plane t gets its +z, 0 and -z contributions at source planes t-1, t, t+1; the partial
+//    sets wait in registers and plane t is accumulated in the reference order i = 0..Q-1.
+//  * NBUF = 2 double-buffers the LDS exchange so each plane needs one barrier; NBUF = 1 uses
+//    half the LDS and two barriers per plane.
+//  * The y boundary planes of the jet/RTI cases are written by the lanes that finish rows 1 and
+//    ny, so a step is one launch.
+//  * Blocks are remapped so that each XCD gets a contiguous range of tiles.
+#pragma once
+
+#include <cstdlib>
+
+#include "kernels.hpp"
+#include "march_core.hpp"
+
+#ifndef CGLBM_MARCH_PREFETCH
+#define CGLBM_MARCH_PREFETCH 1
+#endif
+#ifndef CGLBM_MARCH_WAVES
+#define CGLBM_MARCH_WAVES 0
+#endif
+
+#if CGLBM_MARCH_WAVES > 0
+#define CGLBM_MARCH_OCC __attribute__((amdgpu_waves_per_eu(CGLBM_MARCH_WAVES, CGLBM_MARCH_WAVES)))
+#else
+#define CGLBM_MARCH_OCC
+#endif
+
+namespace cglbm {
+namespace march {
+
+constexpr int TX = 32;
+constexpr int kPsiSlots = 5;
+
+__host__ __device__ constexpr bool cross_col(int i) { return !(kCx[i] == 0 && kCy[i] == 0); }
+__host__ __device__ constexpr int slot_of(int i) {
+    int s = 0;
+    for (int j = 0; j < i; ++j) s += cross_col(j) ? 1 : 0;
+    return s;
+}
+template <int Q>
+__host__ __device__ constexpr int nslots() {
+    int s = 0;
+    for (int j = 0; j < Q; ++j) s += cross_col(j) ? 1 : 0;
+    return s;
+}
+
+// The exchange planes carry a one-cell margin so a tile source can push every direction
+// without a bounds test (pushes that leave the tile land in the margin and are never read).
+template <int Q, int TY, int NBUF>
+struct Shared {
+    float2 xfer[NBUF][nslots<Q>()][TY + 2][TX + 2];
+    float psi[kPsiSlots][TY + 4][TX + 4];
+};
+
+template <int Q, int TY, int NBUF>
+constexpr size_t shared_bytes() {
+    return sizeof(Shared<Q, TY, NBUF>);
+}
+
+template <int Q>
+struct Held {
+    float2 v[Q];
+};
+
+// One state as a buffer resource: field f of cell `id` at byte f * fs4 + 4 * id. Loads and
+// stores take the cell offset in a VGPR and the field offset in an SGPR, so the 11 accesses of a
+// cell need no per-field address arithmetic.
+struct FieldBuf {
+    __amdgpu_buffer_rsrc_t r;
+    uint32_t fs4;
+    __device__ __forceinline__ FieldBuf(const float *base, uint32_t fs, uint32_t span) {
+        r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), static_cast<short>(0), static_cast<int>(span),
+                                              0x00020000);
+        fs4 = fs * 4u;
+    }
+    __device__ __forceinline__ float ld(int f, uint32_t id) const {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, id * 4u, static_cast<uint32_t>(f) * fs4, 0));
+    }
+    __device__ __forceinline__ void st(int f, uint32_t id, float v) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, id * 4u, static_cast<uint32_t>(f) * fs4, 0);
+    }
+    __device__ __forceinline__ Mom mom(uint32_t id) const {
+        Mom m;
+        m.rr = ld(0, id);
+        m.rb = ld(1, id);
+        m.ux = ld(2, id);
+        m.uy = ld(3, id);
+        m.uz = ld(4, id);
+        m.pxx = ld(5, id);
+        m.pxy = ld(6, id);
+        m.pyy = ld(7, id);
+        m.pyz = ld(8, id);
+        m.pzz = ld(9, id);
+        m.pxz = ld(10, id);
+        return m;
+    }
+};
+
+// Boundary planes of the new state (jet/boundary_conditions.cuh:16-109 or the wall copies of
+// rti/rtiIn.cuh:77-119), written by the lane that has just finished row y = 1 (inlet/bottom) or
+// y = ny (outlet/top) of column (gx, zt).
+__device__ __forceinline__ void boundary_planes(const MarchArgs &a, const FieldBuf &A, const FieldBuf &B, int gx,
+                                                int zt, int rown, const float out[11]) {
+    auto cell = [&](int row) {
+        return static_cast<uint32_t>(gx) + a.py * static_cast<uint32_t>(row) + a.pz * static_cast<uint32_t>(zt);
+    };
+    if (rown == 0) {
+        const uint32_t i0 = cell(0);
+        if (a.ybc == kYWalls) {
+            B.st(0, i0, out[0]);
+            B.st(1, i0, out[1]);
+            B.st(2, i0, 0.0f);
+            B.st(3, i0, 0.0f);
+            B.st(4, i0, 0.0f);
+        } else {
+            const float dx = static_cast<float>(gx) - a.jet_x0;
+            const float dz = static_cast<float>(zt) - a.jet_z0;
+            const float m = ((dx * dx + dz * dz) <= (a.jet_r * a.jet_r)) ? 1.0f : 0.0f;
+            float U = a.jet_u, Ux = 0.0f, Uz = 0.0f;
+            const bool noisy = a.jet_noise != 0.0f;
+            if (noisy) {
+                const float eps = a.jet_noise;
+                const float q0 = inlet_uniform(a.seed, gx, zt, a.step, 0);
+                const float q1 = inlet_uniform(a.seed, gx, zt, a.step, 1);
+                const float q2 = inlet_uniform(a.seed, gx, zt, a.step, 2);
+                Ux = U * (eps * (2.0f * q1 - 1.0f));
+                Uz = U * (eps * (2.0f * q2 - 1.0f));
+                U = U * (1.0f + eps * (2.0f * q0 - 1.0f));
+                Ux = m * Ux;
+                Uz = m * Uz;
+            }
+            B.st(0, i0, (1.0f - m) * 1.0f);
+            B.st(1, i0, m * 1.0f);
+            B.st(2, i0, noisy ? Ux : 0.0f);
+            B.st(3, i0, m * U);
+            B.st(4, i0, noisy ? Uz : 0.0f);
+        }
+#pragma unroll
+        for (int f = 5; f < 11; ++f) B.st(f, i0, out[f]);
+    }
+    if (rown == a.ny - 1) {
+        const uint32_t iT = cell(a.ny + 1);
+        if (a.ybc == kYWalls) {
+            B.st(0, iT, out[0]);
+            B.st(1, iT, out[1]);
+            B.st(2, iT, 0.0f);
+            B.st(3, iT, 0.0f);
+            B.st(4, iT, 0.0f);
+        } else {
+            const float rrO = A.ld(0, iT), rbO = A.ld(1, iT);
+            const float uxO = A.ld(2, iT), uyO = A.ld(3, iT), uzO = A.ld(4, iT);
+            const float uyF = out[3];
+            const float uc = a.outlet == 1 ? 1.0f : (uyF < 0.0f ? 0.0f : (uyF > 1.0f ? 1.0f : uyF));
+            const float rhoO = rrO + rbO;
+            if (rhoO <= 1.0e-20f) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) B.st(f, iT, out[f]);
+            } else {
+                B.st(0, iT, rrO - uc * (rrO - out[0]));
+                B.st(1, iT, rbO - uc * (rbO - out[1]));
+                B.st(2, iT, uxO - uc * (uxO - out[2]));
+                B.st(3, iT, uyO - uc * (uyO - out[3]));
+                B.st(4, iT, uzO - uc * (uzO - out[4]));
+            }
+        }
+#pragma unroll
+        for (int f = 5; f < 11; ++f) B.st(f, iT, out[f]);
+    }
+}
+
+enum : int { kRoleOwn = 0, kRoleRows = 1, kRoleCols = 2, kRoleNone = 3 };
+
+template <int Q, int YBC, int TY, int NBUF, bool FORCED>
+__global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
+    march_kernel(MarchArgs a, int ntx, int nty, int nchunk, int nblocks) {
+    constexpr int NOWN = TX * TY;
+    constexpr int PX = TX + 4, PY = TY + 4, NPSI = PX * PY;
+    constexpr int NPASS = (NPSI + 127) / 128;  // psi cells per ring lane
+    constexpr uint32_t ALL = pair_mask_all<Q>();
+    constexpr uint32_t MX = pair_mask_where<Q>(0), MY = pair_mask_where<Q>(1), MZ = pair_mask_where<Q>(2);
+
+    __shared__ Shared<Q, TY, NBUF> sh;
+
+    const int b = blockIdx.x;
+    const int per = gridDim.x / 8;
+    const int logical = (b % 8) * per + b / 8;  // XCD-contiguous tile ranges
+    if (logical >= nblocks) return;
+    const int tix = logical % ntx;
+    const int tiy = (logical / ntx) % nty;
+    const int chunk = logical / (ntx * nty);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int z0 = static_cast<int>((static_cast<int64_t>(chunk) * nz) / nchunk);
+    const int z1 = static_cast<int>((static_cast<int64_t>(chunk + 1) * nz) / nchunk);
+    const int Lc = z1 - z0;
+    const int x0 = tix * TX, r0 = tiy * TY;
+    const int t = threadIdx.x;
+    const Coeffs c = a.c;
+    const FieldBuf Abuf(a.A0, a.fsA, a.spanA);
+    const FieldBuf Bbuf(a.B0, a.fsB, a.spanB);
+
+    // ---- role and source cell of this lane -----------------------------------------------
+    int role, sx, sy;
+    if (t < NOWN) {
+        role = kRoleOwn;
+        sx = t % TX;
+        sy = t / TX;
+    } else if (t < NOWN + 64) {
+        const int l = t - NOWN;
+        role = kRoleRows;
+        sx = l % TX;
+        sy = l < TX ? -1 : TY;
+    } else {
+        const int l = t - NOWN - 64;
+        role = kRoleCols;
+        if (l < TY) {
+            sx = -1;
+            sy = l;
+        } else if (l < 2 * TY) {
+            sx = TX;
+            sy = l - TY;
+        } else if (l < 2 * TY + 4) {
+            const int k = l - 2 * TY;
+            sx = (k & 1) ? TX : -1;
+            sy = (k & 2) ? TY : -1;
+        } else {
+            role = kRoleNone;
+            sx = 0;
+            sy = 0;
+        }
+    }
+    const bool ring = t >= NOWN;
+    const int gx = ((x0 + sx) % nx + nx) % nx;
+    const int rsrc = r0 + sy;  // interior row of the source
+    int srow = -1;             // stored row of the source, -1 = not needed
+    bool sghost = false;
+    if (role != kRoleNone) {
+        if constexpr (YBC == kYPeriodic) {
+            srow = ((rsrc % ny) + ny) % ny;
+        } else if (rsrc >= -1 && rsrc <= ny) {
+            srow = rsrc + 1;
+            sghost = (rsrc == -1 || rsrc == ny);
+        }
+    }
+    const int frow = sghost ? (rsrc == -1 ? 1 : ny) : 0;  // adjacent interior row of a ghost source
+    const bool own_target = role == kRoleOwn && (x0 + sx < nx) && (rsrc < ny);
+
+    auto cell = [&](int x, int row, int z) -> uint32_t {
+        return static_cast<uint32_t>(x) + a.py * static_cast<uint32_t>(row) + a.pz * static_cast<uint32_t>(z);
+    };
+    auto wrapz = [&](int z) { return ((z % nz) + nz) % nz; };
+
+    // ---- psi cells of a ring lane ----------------------------------------------------------
+    int psi_x[NPASS], psi_row[NPASS];
+    float psi_r[NPASS], psi_b[NPASS];
+#pragma unroll
+    for (int k = 0; k < NPASS; ++k) {
+        const int ci = (t - NOWN) + k * 128;
+        const int px = ci % PX - 2, py = ci / PX - 2;
+        psi_x[k] = ((x0 + px) % nx + nx) % nx;
+        const int rr_ = r0 + py;
+        if (!ring || ci >= NPSI) psi_row[k] = -2;
+        else if constexpr (YBC == kYPeriodic) psi_row[k] = ((rr_ % ny) + ny) % ny;
+        else psi_row[k] = (rr_ >= -1 && rr_ <= ny) ? rr_ + 1 : -1;
+        psi_r[k] = 0.0f;
+        psi_b[k] = 1.0f;
+    }
+    auto psi_load = [&](int z) {
+        const int zz = wrapz(z);
+#pragma unroll
+        for (int k = 0; k < NPASS; ++k) {
+            if (psi_row[k] >= 0) {
+                const uint32_t id = cell(psi_x[k], psi_row[k], zz);
+                psi_r[k] = Abuf.ld(0, id);
+                psi_b[k] = Abuf.ld(1, id);
+            }
+        }
+    };
+    auto psi_store = [&](int slot) {
+#pragma unroll
+        for (int k = 0; k < NPASS; ++k) {
+            if (psi_row[k] > -2) {
+                const int ci = (t - NOWN) + k * 128;
+                const float v = psi_row[k] >= 0 ? (psi_r[k] - psi_b[k]) / (psi_r[k] + psi_b[k]) : 0.0f;
+                sh.psi[slot][ci / PX][ci % PX] = v;
+            }
+        }
+    };
+    auto psi_slot = [&](int z) { return (z - (z0 - 2)) % kPsiSlots; };
+
+    // psi of neighbour c_j of this lane's source (forces.cuh:43-50; bounded y limited as
+    // mathUtilities.cuh:25-33) for source plane p.
+    auto psi_reader = [&](int p) {
+        const int sm = psi_slot(p - 1), s0 = psi_slot(p), sp = psi_slot(p + 1);
+        return [&, sm, s0, sp](auto J) -> float {
+            constexpr int j = decltype(J)::value;
+            constexpr int cx = kCx[j], cy = kCy[j], cz = kCz[j];
+            int prow = sy + cy;
+            if constexpr (YBC != kYPeriodic) {
+                if constexpr (cy > 0) {
+                    if (rsrc + cy > ny - 1) prow = (ny - 1) - r0;
+                }
+            }
+            const int slot = cz == 0 ? s0 : (cz > 0 ? sp : sm);
+            return sh.psi[slot][prow + 2][sx + cx + 2];
+        };
+    };
+
+    Held<Q> H0, H1, H2;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) H0.v[i] = H1.v[i] = H2.v[i] = make_float2(0.0f, 0.0f);
+
+    // ---- prologue: psi planes z0-2 .. z0+1 (ring lanes), first moments ---------------------
+    if (ring) {
+        for (int q = 0; q < 4; ++q) {
+            psi_load(z0 - 2 + q);
+            psi_store(q);
+        }
+    }
+    Mom nxt{};
+#if CGLBM_MARCH_PREFETCH
+    if (srow >= 0) nxt = Abuf.mom(cell(gx, srow, wrapz(z0 - 1)));
+#endif
+    __syncthreads();
+
+    const int kend = Lc + 1;
+    for (int k = 0; k <= kend; ++k) {
+        const int p = z0 - 1 + k;  // source plane of this step; target plane p - 1 completes
+        const int pz = wrapz(p);
+        const bool edge = (k == 0) || (k == kend);
+        const int xb = NBUF == 2 ? (k & 1) : 0;
+        Held<Q> &HA = H0;  // target p - 1
+        Held<Q> &HB = H1;  // target p
+        Held<Q> &HC = H2;  // target p + 1
+#if CGLBM_MARCH_PREFETCH
+        const Mom cur = nxt;
+        if (k < kend && srow >= 0) nxt = Abuf.mom(cell(gx, srow, wrapz(p + 1)));
+#else
+        Mom cur{};
+        if (srow >= 0) cur = Abuf.mom(cell(gx, srow, pz));
+#endif
+        Mom curF{};
+        if constexpr (YBC != kYPeriodic) {
+            if (sghost) curF = Abuf.mom(cell(gx, frow, pz));
+        }
+
+        // ---- sources ----------------------------------------------------------------------
+        if (srow >= 0) {
+            auto lds_push = [&](auto I, float fr, float fb) {
+                constexpr int i = decltype(I)::value;
+                if constexpr (cross_col(i)) {
+                    const int txx = sx + kCx[i], tyy = sy + kCy[i];
+                    if (txx >= -1 && txx <= TX && tyy >= -1 && tyy <= TY)
+                        sh.xfer[xb][slot_of(i)][tyy + 1][txx + 1] = make_float2(fr, fb);
+                }
+            };
+            auto own_sink = [&](auto I, float fr, float fb) {
+                constexpr int i = decltype(I)::value;
+                if constexpr (cross_col(i)) {
+                    sh.xfer[xb][slot_of(i)][sy + kCy[i] + 1][sx + kCx[i] + 1] = make_float2(fr, fb);
+                } else if constexpr (kCz[i] == 0) {
+                    HB.v[i] = make_float2(fr, fb);
+                } else if constexpr (kCz[i] > 0) {
+                    HC.v[i] = make_float2(fr, fb);
+                } else {
+                    HA.v[i] = make_float2(fr, fb);
+                }
+            };
+            auto run = [&](auto MASKC, auto RESTC, auto &sink) {
+                constexpr uint32_t M = decltype(MASKC)::value;
+                constexpr bool R = decltype(RESTC)::value;
+                if (YBC != kYPeriodic && sghost) {
+                    emit_ghost<Q, M, R>(cur, curF, c, sink);
+                } else {
+                    Iface f = iface_begin(cur);
+                    if (f.on) {
+                        colour_gradient<Q>(psi_reader(p), f.Fx, f.Fy, f.Fz);
+                        iface_finish(f, cur, c, a.conservative);
+                        emit_regular<Q, M, R, FORCED, true>(cur, c, f, sink);
+                    } else {
+                        emit_regular<Q, M, R, FORCED, false>(cur, c, f, sink);
+                    }
+                }
+            };
+            if (role == kRoleOwn) {
+                if (edge) run(std::integral_constant<uint32_t, MZ>{}, std::false_type{}, own_sink);
+                else run(std::integral_constant<uint32_t, ALL>{}, std::true_type{}, own_sink);
+            } else if (role == kRoleRows) {
+                if (edge) run(std::integral_constant<uint32_t, MY & MZ>{}, std::false_type{}, lds_push);
+                else run(std::integral_constant<uint32_t, MY>{}, std::false_type{}, lds_push);
+            } else {
+                if (edge) run(std::integral_constant<uint32_t, MX & MZ>{}, std::false_type{}, lds_push);
+                else run(std::integral_constant<uint32_t, MX>{}, std::false_type{}, lds_push);
+            }
+        }
+        __syncthreads();
+
+        // ---- ring lanes: psi two planes ahead (plane p + 3, first read at step k + 2) ------
+        if (ring && k + 3 <= kend + 1) {
+            psi_load(p + 3);
+            psi_store(psi_slot(p + 3));
+        }
+
+        // ---- targets ----------------------------------------------------------------------
+        if (role == kRoleOwn) {
+            unroll<0, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                if constexpr (cross_col(i)) {
+                    const float2 v = sh.xfer[xb][slot_of(i)][sy + 1][sx + 1];
+                    if constexpr (kCz[i] == 0) HB.v[i] = v;
+                    else if constexpr (kCz[i] > 0) HC.v[i] = v;
+                    else HA.v[i] = v;
+                }
+            });
+            if (k >= 2 && own_target) {
+                const int zt = wrapz(p - 1);
+                float out[11];
+                accumulate<Q, FORCED>([&](auto I) { return HA.v[decltype(I)::value]; }, c, out);
+                const uint32_t id = cell(gx, srow, zt);
+#pragma unroll
+                for (int f = 0; f < 11; ++f) Bbuf.st(f, id, out[f]);
+                if constexpr (YBC != kYPeriodic) boundary_planes(a, Abuf, Bbuf, gx, zt, rsrc, out);
+            }
+            H0 = H1;  // target p becomes p - 1, p + 1 becomes p
+            H1 = H2;
+        }
+        if constexpr (NBUF == 1) __syncthreads();
+    }
+}
+
+template <int Q, int YBC, int TY, int NBUF>
+hipError_t launch_shape(const MarchArgs &a, int L, hipStream_t s) {
+    const int ntx = (a.nx + TX - 1) / TX;
+    const int nty = (a.ny + TY - 1) / TY;
+    const int nchunk = (a.nz + L - 1) / L;
+    const int nblocks = ntx * nty * nchunk;
+    const int grid = (nblocks + 7) / 8 * 8;
+    // The body-force variant exists for bounded y (the RTI walls); a periodic forced domain takes
+    // the gather path (march_supports).
+    if constexpr (YBC != kYPeriodic) {
+        if (a.forced) {
+            march_kernel<Q, YBC, TY, NBUF, true><<<grid, TX * TY + 128, 0, s>>>(a, ntx, nty, nchunk, nblocks);
+            return hipGetLastError();
+        }
+    }
+    march_kernel<Q, YBC, TY, NBUF, false><<<grid, TX * TY + 128, 0, s>>>(a, ntx, nty, nchunk, nblocks);
+    return hipGetLastError();
+}
+
+// Double-buffered exchange must fit the 160 KB LDS of a CU.
+template <int Q, int TY>
+constexpr bool fits2() {
+    return shared_bytes<Q, TY, 2>() <= 160 * 1024;
+}
+
+template <int Q, int YBC>
+hipError_t launch_q(const MarchArgs &a, MarchShape m, hipStream_t s) {
+    const bool one = m.nbuf == 1;
+    switch (m.ty) {
+    case 4: return one ? launch_shape<Q, YBC, 4, 1>(a, m.L, s) : launch_shape<Q, YBC, 4, 2>(a, m.L, s);
+    case 16:
+        if constexpr (fits2<Q, 16>()) {
+            if (!one) return launch_shape<Q, YBC, 16, 2>(a, m.L, s);
+        }
+        return launch_shape<Q, YBC, 16, 1>(a, m.L, s);
+    default: return one ? launch_shape<Q, YBC, 8, 1>(a, m.L, s) : launch_shape<Q, YBC, 8, 2>(a, m.L, s);
+    }
+}
+
+template <int Q>
+hipError_t launch_march_q(const MarchArgs &a, MarchShape m, hipStream_t s) {
+    return a.ybc == kYPeriodic ? launch_q<Q, kYPeriodic>(a, m, s) : launch_q<Q, kYJet>(a, m, s);
+}
+
+}  // namespace march
+}  // namespace cglbm

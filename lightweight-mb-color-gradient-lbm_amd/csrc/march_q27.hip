@@ -1,0 +1,6 @@
+// D3Q27 instantiations of the z-marching step (march_kernel.inl).
+#include "march_kernel.inl"
+
+namespace cglbm {
+hipError_t launch_march_q27(const MarchArgs &a, MarchShape m, hipStream_t s) { return march::launch_march_q<27>(a, m, s); }
+}  // namespace cglbm

@@ -52,12 +52,25 @@ __global__ void __launch_bounds__(kThreads) reduce_partials_kernel(const float *
     if (threadIdx.x < 6) partials[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void reduce_final_kernel(const double *partials, int nblocks, double *result) {
-    if (threadIdx.x < 6) {
-        double s = 0.0;
-        for (int b = 0; b < nblocks; ++b) s += partials[b * 6 + threadIdx.x];
-        result[threadIdx.x] = s;
+// One block: lane l sums partials l, l + 256, ... in order, then a fixed tree combines lanes.
+__global__ void __launch_bounds__(kThreads) reduce_final_kernel(const double *partials, int nblocks, double *result) {
+    __shared__ double sh[6][kThreads];
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblocks; b += kThreads) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s[k] += partials[b * 6 + k];
     }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) result[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 }  // namespace
@@ -66,7 +79,7 @@ hipError_t launch_reduce(const float *const *A, int nx, int ny, int nz, int y0, 
                          double *partials, double *result, hipStream_t s) {
     reduce_partials_kernel<<<kReduceBlocks, kThreads, 0, s>>>(A[0], A[1], A[2], A[3], A[4], nx, ny, nz, y0, py, pz,
                                                               partials);
-    reduce_final_kernel<<<1, 64, 0, s>>>(partials, kReduceBlocks, result);
+    reduce_final_kernel<<<1, kThreads, 0, s>>>(partials, kReduceBlocks, result);
     return hipGetLastError();
 }
 

@@ -13,7 +13,10 @@
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <array>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -148,15 +151,37 @@ int check_moments(const cglbm_moments *m, const char *name) {
     return CGLBM_OK;
 }
 
+namespace {
+std::mutex g_tuned_mu;
+std::map<std::array<int, 5>, MarchShape> g_tuned;
+std::array<int, 5> tuned_key(const cglbm_config &c) { return {c.q, c.ybc, c.nx, c.ny, c.nz}; }
+}  // namespace
+
+bool tuned_shape_lookup(const cglbm_config &c, MarchShape *out) {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    auto it = g_tuned.find(tuned_key(c));
+    if (it == g_tuned.end()) return false;
+    *out = it->second;
+    return true;
+}
+
+void tuned_shape_store(const cglbm_config &c, MarchShape m) {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    g_tuned[tuned_key(c)] = m;
+}
+
 int launch_step(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
-                const cglbm_layout *L, int64_t step, int kernel, hipStream_t s) {
+                const cglbm_layout *L, int64_t step, int kernel, const MarchShape *shape, hipStream_t s) {
     StepArgs a = make_step_args(A, B, cfg, L, step);
     const bool march_ok = march_supports(a, cfg->q);
     if (kernel == CGLBM_KERNEL_MARCH && !march_ok)
         return fail(CGLBM_EINVAL, "the z-marching kernel does not support this shape/layout");
     hipError_t e;
     if (kernel != CGLBM_KERNEL_GATHER && march_ok) {
-        e = launch_march(a, cfg->q, s);
+        MarchShape m;
+        if (shape && shape->ty > 0) m = *shape;
+        else if (!tuned_shape_lookup(*cfg, &m)) m = march_default_shape(a, cfg->q);
+        e = launch_march_shape(a, cfg->q, m, s);
         if (e != hipSuccess) return hip_fail(e, "launch_march");
         return CGLBM_OK;  // boundary planes are written inside the marching step
     }
@@ -199,7 +224,7 @@ int cglbm_launch_rcs(const cglbm_moments *A, const cglbm_moments *B, const cglbm
     if (int rc = check_moments(B, "B")) return rc;
     if (A->rhor == B->rhor) return fail(CGLBM_EINVAL, "A and B alias");
     if (kernel < CGLBM_KERNEL_AUTO || kernel > CGLBM_KERNEL_MARCH) return fail(CGLBM_EINVAL, "unknown kernel %d", kernel);
-    return launch_step(A, B, cfg, layout, step, kernel, static_cast<hipStream_t>(stream));
+    return launch_step(A, B, cfg, layout, step, kernel, nullptr, static_cast<hipStream_t>(stream));
 }
 
 // launch_JetBoundaryConditions (launch.cuh:78-83).

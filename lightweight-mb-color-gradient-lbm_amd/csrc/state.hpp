@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "cglbm.h"
+#include "kernels.hpp"
 #include "step_common.hpp"
 
 namespace cglbm {
@@ -15,8 +16,13 @@ StepArgs make_step_args(const cglbm_moments *A, const cglbm_moments *B, const cg
                         const cglbm_layout *L, int64_t step);
 BcArgs make_bc_args(const cglbm_moments *B, const cglbm_moments *Aold, const cglbm_config *c,
                     const cglbm_layout *L, int64_t step);
+// shape == nullptr: the tuned shape recorded for this configuration, else the default.
 int launch_step(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
-                const cglbm_layout *L, int64_t step, int kernel, hipStream_t s);
+                const cglbm_layout *L, int64_t step, int kernel, const MarchShape *shape, hipStream_t s);
+
+// Process-wide record of autotuned z-marching shapes, keyed by (q, ybc, nx, ny, nz).
+bool tuned_shape_lookup(const cglbm_config &c, MarchShape *out);
+void tuned_shape_store(const cglbm_config &c, MarchShape m);
 
 inline int stored_ny(const cglbm_config &c) { return c.ybc == CGLBM_Y_PERIODIC ? c.ny : c.ny + 2; }
 inline int64_t stored_cells(const cglbm_config &c) {

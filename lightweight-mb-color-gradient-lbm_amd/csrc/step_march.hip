@@ -1,10 +1,80 @@
-// z-marching fused step (placeholder until the tuned kernel lands).
+// Dispatch of the z-marching step (kernels in march_kernel.inl, instantiated per stencil in
+// march_q19.hip and march_q27.hip) and its default tile shapes.
+#include <cstdlib>
+
 #include "kernels.hpp"
 
 namespace cglbm {
 
-bool march_supports(const StepArgs &, int) { return false; }
+namespace {
+int env_int(const char *name, int lo, int hi) {
+    const char *e = std::getenv(name);
+    const int v = e ? std::atoi(e) : 0;
+    return (v >= lo && v <= hi) ? v : 0;
+}
+}  // namespace
 
-hipError_t launch_march(const StepArgs &, int, hipStream_t) { return hipErrorInvalidValue; }
+MarchShape march_default_shape(const StepArgs &a, int q) {
+    // CGLBM_MARCH_TY / CGLBM_MARCH_L / CGLBM_MARCH_NBUF override the defaults for tuning sweeps.
+    MarchShape m{8, 16, q == 19 ? 2 : 1};
+    if (const int ty = env_int("CGLBM_MARCH_TY", 1, 64)) m.ty = ty;
+    if (const int L = env_int("CGLBM_MARCH_L", 1, 4096)) m.L = L;
+    if (const int nb = env_int("CGLBM_MARCH_NBUF", 1, 2)) m.nbuf = nb;
+    if (m.L > a.nz) m.L = a.nz;
+    return m;
+}
+
+bool march_args(const StepArgs &a, MarchArgs &m) {
+    const int64_t fsA = a.A[1] - a.A[0], fsB = a.B[1] - a.B[0];
+    if (fsA <= 0 || fsB <= 0) return false;
+    for (int f = 2; f < 11; ++f)
+        if (a.A[f] - a.A[0] != fsA * f || a.B[f] - a.B[0] != fsB * f) return false;
+    // last addressed cell of a field, in elements
+    const int64_t nyt = a.ybc == kYPeriodic ? a.ny : a.ny + 2;
+    const int64_t last = (a.nx - 1) + int64_t(a.py) * (nyt - 1) + int64_t(a.pz) * (a.nz - 1);
+    const int64_t spanA = (fsA * 10 + last + 1) * 4, spanB = (fsB * 10 + last + 1) * 4;
+    if (spanA >= (int64_t(1) << 32) || spanB >= (int64_t(1) << 32)) return false;
+    m.A0 = a.A[0];
+    m.B0 = a.B[0];
+    m.fsA = static_cast<uint32_t>(fsA);
+    m.fsB = static_cast<uint32_t>(fsB);
+    m.spanA = static_cast<uint32_t>(spanA);
+    m.spanB = static_cast<uint32_t>(spanB);
+    m.nx = a.nx;
+    m.ny = a.ny;
+    m.nz = a.nz;
+    m.py = a.py;
+    m.pz = a.pz;
+    m.ybc = a.ybc;
+    m.outlet = a.outlet;
+    m.conservative = a.conservative;
+    m.forced = a.forced;
+    m.c = a.c;
+    m.jet_x0 = a.jet_x0;
+    m.jet_z0 = a.jet_z0;
+    m.jet_r = a.jet_r;
+    m.jet_u = a.jet_u;
+    m.jet_noise = a.jet_noise;
+    m.seed = a.seed;
+    m.step = a.step;
+    return true;
+}
+
+bool march_supports(const StepArgs &a, int q) {
+    if (a.forced && a.ybc == kYPeriodic) return false;  // no periodic body-force variant
+    MarchArgs m;
+    return (q == 19 || q == 27) && a.nx >= 3 && a.ny >= 3 && a.nz >= 3 && march_args(a, m);
+}
+
+hipError_t launch_march_shape(const StepArgs &a, int q, MarchShape m, hipStream_t s) {
+    if ((m.ty != 4 && m.ty != 8 && m.ty != 16) || m.L < 1 || (m.nbuf != 1 && m.nbuf != 2)) return hipErrorInvalidValue;
+    MarchArgs ma;
+    if (!march_args(a, ma)) return hipErrorInvalidValue;
+    return q == 19 ? launch_march_q19(ma, m, s) : launch_march_q27(ma, m, s);
+}
+
+hipError_t launch_march(const StepArgs &a, int q, hipStream_t s) {
+    return launch_march_shape(a, q, march_default_shape(a, q), s);
+}
 
 }  // namespace cglbm

@@ -150,3 +150,40 @@ def test_launch_level_reference_layout(oracle):
         carrier.sync()
         got = carrier.get_state()
     np.testing.assert_array_equal(got[:, 1:-1, :, 1:-1], want)
+
+
+SHAPES = [(4, 3, 1), (8, 5, 2), (16, 7, 1), (16, 16, 2), (8, 1, 2), (4, 100, 2)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"ty{a}L{b}nb{c}" for a, b, c in SHAPES])
+@pytest.mark.parametrize("name", ["jet_small_q19", "jet_small_q27", "bubble_q19", "rand55_q27"])
+def test_march_shapes_match_reference_golden(oracle, name, shape):
+    """Every tile shape of the z-marching kernel (tile rows, chunk length, exchange buffers)
+    reproduces the reference's golden vectors bit for bit."""
+    cg = load_product()
+    z, meta = _golden.load(name)
+    p, A = _golden.initial_state(oracle, z, meta)
+    ty, L, nbuf = shape
+    print(f"case {name} shape {shape}", flush=True)
+    with cg.Domain(_cfg(cg, p), kernel=cg.KERNEL_MARCH) as dom:
+        dom.set_tile(ty, min(L, p.nz), nbuf)
+        dom.set_state(A)
+        k = 0
+        for s in meta["steps"][:2]:
+            dom.step(s - k)
+            k = s
+            shas = _golden.field_shas(dom.get_state())
+            assert shas == list(z[f"sha_{s}"]), f"step {s} differs"
+
+
+def test_autotune_keeps_state_and_parity(oracle):
+    cg = load_product()
+    z, meta = _golden.load("bubble_q27")
+    p, A = _golden.initial_state(oracle, z, meta)
+    with cg.Domain(_cfg(cg, p)) as dom:
+        dom.set_state(A)
+        ty, L, nb, ms = dom.autotune(2)
+        assert ty in (4, 8, 16) and 1 <= L <= p.nz and nb in (1, 2) and ms > 0
+        np.testing.assert_array_equal(dom.get_state(), A)  # autotuning does not advance the state
+        dom.step(10)
+        assert _golden.field_shas(dom.get_state()) == list(z["sha_10"])
