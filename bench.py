@@ -34,14 +34,18 @@ def _load():
     return ge._import_package()
 
 
-def measure(cg, q: int, n: int, steps: int, warmup: int, kernel: int):
+def measure(cg, q: int, n: int, steps: int, warmup: int, kernel: int, tile=None):
     cfg = cg.Config(nx=n, ny=n, nz=n, q=q, tau=0.8, sigma=0.01, beta=0.7)
     with cg.Domain(cfg, kernel=kernel) as dom:
         dom.init_bubble(radius=24.0 * n / 128.0, width=4.0)
         m0 = dom.reduce()
         shape = None
         if kernel != cg.KERNEL_GATHER:
-            shape = dom.autotune(3)[:3]  # tile rows, chunk length, exchange buffers
+            if tile:
+                dom.set_tile(*tile)
+                shape = tuple(tile)
+            else:
+                shape = dom.autotune(3)[:3]  # tile rows, chunk length, exchange buffers
         dom.step(warmup)
         dom.sync()
         ms = dom.time_steps(steps)
@@ -137,6 +141,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="auto", choices=["auto", "gather", "march"])
     ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--tile19", default=None, help="fixed D3Q19 tile 'rows,chunk,buffers' instead of autotuning")
+    ap.add_argument("--tile27", default=None, help="fixed D3Q27 tile 'rows,chunk,buffers' instead of autotuning")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -147,9 +153,11 @@ def main() -> int:
     cg = _load()
     kern = {"auto": cg.KERNEL_AUTO, "gather": cg.KERNEL_GATHER, "march": cg.KERNEL_MARCH}[args.kernel]
     print(f"bench: D3Q19 {args.n}^3 warmup {args.warmup} steps {args.steps}", file=sys.stderr, flush=True)
-    r19 = measure(cg, 19, args.n, args.steps, args.warmup, kern)
+    t19 = [int(v) for v in args.tile19.split(",")] if args.tile19 else None
+    t27 = [int(v) for v in args.tile27.split(",")] if args.tile27 else None
+    r19 = measure(cg, 19, args.n, args.steps, args.warmup, kern, t19)
     print(f"bench: D3Q19 {r19['mlups']:.0f} MLUPS; D3Q27 next", file=sys.stderr, flush=True)
-    r27 = measure(cg, 27, args.n, args.steps, args.warmup, kern)
+    r27 = measure(cg, 27, args.n, args.steps, args.warmup, kern, t27)
     print(f"bench: D3Q27 {r27['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
     if args.calibrate_copy:
         calibrate_copy(cg, args.calibrate_copy)
