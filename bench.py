@@ -88,6 +88,53 @@ def calibrate_copy(cg, mib: int):
     print(f"bench: calibration copy of {mib} MiB done", file=sys.stderr, flush=True)
 
 
+def batched_workload(cg, seed: int = 0, count: int = 256):
+    """Config 4: `count` independent periodic D3Q19 cubes, edges uniform in {32, 48, 64, 80, 96},
+    drop radius uniform in [4, min(24, e/3)], tau uniform in [0.52, 1.0], sigma log-uniform in
+    [1e-3, 2e-2], beta 0.7 (seeded)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        e = int(rng.choice([32, 48, 64, 80, 96]))
+        r = float(rng.uniform(4.0, min(24.0, e / 3.0)))
+        tau = float(rng.uniform(0.52, 1.0))
+        sigma = float(np.exp(rng.uniform(np.log(1e-3), np.log(2e-2))))
+        out.append((cg.Config(nx=e, ny=e, nz=e, q=19, tau=tau, sigma=sigma, beta=0.7), r))
+    return out
+
+
+def measure_batched(cg, steps: int, seq_steps: int, bundle: int = 10):
+    work = batched_workload(cg)
+    cells = sum(c.interior_cells for c, _ in work)
+    doms = []
+    for cfg, r in work:
+        d = cg.Domain(cfg)
+        d.init_bubble(radius=r, width=4.0)
+        doms.append(d)
+    b = cg.Batch()
+    for d in doms:
+        b.add(d)
+    b.autotune(1)
+    b.set_graph(bundle)
+    b.step(2 * bundle)
+    b.sync()
+    ms = b.time_steps(steps)
+    ms_seq = b.time_sequential(seq_steps)
+    for d in doms:
+        b.remove(d)
+        d.close()
+    b.close()
+    mlups = cells * steps / (ms / 1e3) / 1e6
+    mlups_seq = cells * seq_steps / (ms_seq / 1e3) / 1e6
+    return {"domains": len(work), "cells": cells, "steps": steps, "ms_per_step": round(ms / steps, 4),
+            "value": round(mlups, 1), "unit": "MLUPS", "graph_bundle": bundle,
+            "sequential_plain_launches": {"steps": seq_steps, "ms_per_step": round(ms_seq / seq_steps, 4),
+                                          "value": round(mlups_seq, 1)},
+            "speedup_vs_sequential": round(mlups / mlups_seq, 2),
+            "achieved_gbs": round(BYTES_PER_UPDATE_F32 * cells / (ms / steps / 1e3) / 1e9, 1)}
+
+
 def traffic_from_profiles(workload: str):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
@@ -141,6 +188,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel", default="auto", choices=["auto", "gather", "march"])
     ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-domain batched measurement")
+    ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--tile19", default=None, help="fixed D3Q19 tile 'rows,chunk,buffers' instead of autotuning")
     ap.add_argument("--tile27", default=None, help="fixed D3Q27 tile 'rows,chunk,buffers' instead of autotuning")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -159,6 +208,12 @@ def main() -> int:
     print(f"bench: D3Q19 {r19['mlups']:.0f} MLUPS; D3Q27 next", file=sys.stderr, flush=True)
     r27 = measure(cg, 27, args.n, args.steps, args.warmup, kern, t27)
     print(f"bench: D3Q27 {r27['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    batched = None
+    if not args.no_batched:
+        print("bench: 256 batched domains", file=sys.stderr, flush=True)
+        batched = measure_batched(cg, args.batched_steps, 10)
+        print(f"bench: batched {batched['value']:.0f} MLUPS, {batched['speedup_vs_sequential']}x sequential",
+              file=sys.stderr, flush=True)
     if args.calibrate_copy:
         calibrate_copy(cg, args.calibrate_copy)
     workload = f"bubble{args.n}_d3q19"
@@ -190,6 +245,7 @@ def main() -> int:
                   "roofline": {"bound": "hbm", "achieved": round(r27["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": round(r27["achieved_gbs"] / HBM_PEAK_GBS, 4)}},
         "mass_drift_rel": {"d3q19": r19["mass_drift_rel"], "d3q27": r27["mass_drift_rel"]},
+        "batched": batched,
     }
     print(json.dumps(line))
     return 0

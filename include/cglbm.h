@@ -161,6 +161,36 @@ int cglbm_reduce(cglbm_domain *dom, double out[6]);
 /* ASCII VTI with rhor rhob ux uy uz over the stored extent (io/vtiWriter.cuh:60-145). */
 int cglbm_write_vti(cglbm_domain *dom, const char *path);
 
+/* ---- Batches: many domains stepped together ----------------------------------------------
+ * The reference runs one domain per process (main.cu). A batch steps any number of domains with
+ * one launch per kernel class (stencil, bounded y, body force) per step over all their tiles, on
+ * the batch's own stream. Each member's result is identical to stepping it alone. Members keep
+ * every domain entry point (init, copies, diagnostics, VTI); those wait for the batch stream. A
+ * member cannot be stepped on its own or destroyed while it belongs to a batch. */
+
+typedef struct cglbm_batch cglbm_batch;
+
+int cglbm_batch_create(cglbm_batch **out);
+/* Releases the batch (members are detached, not destroyed). */
+int cglbm_batch_destroy(cglbm_batch *b);
+int cglbm_batch_add(cglbm_batch *b, cglbm_domain *dom);
+int cglbm_batch_remove(cglbm_batch *b, cglbm_domain *dom);
+int32_t cglbm_batch_size(const cglbm_batch *b);
+/* One tile shape for every class (ty = 0: per-class autotuned or default). */
+int cglbm_batch_set_tile(cglbm_batch *b, int32_t ty, int32_t L, int32_t nbuf);
+/* Issue steps as replays of a captured HIP graph of `bundle` steps (0 = plain launches). */
+int cglbm_batch_set_graph(cglbm_batch *b, int32_t bundle);
+/* nsteps batched steps of every member (asynchronous on the batch stream). */
+int cglbm_batch_step(cglbm_batch *b, int32_t nsteps);
+/* Same, timed with HIP events on the batch stream. */
+int cglbm_batch_time_steps(cglbm_batch *b, int32_t nsteps, double *ms_total);
+/* Baseline: the members stepped one after another by their own single-domain launches on the
+ * batch stream, timed (used to report the batching speed-up). */
+int cglbm_batch_time_sequential(cglbm_batch *b, int32_t nsteps, double *ms_total);
+/* Autotuner per kernel class (members do not advance). Synchronous. */
+int cglbm_batch_autotune(cglbm_batch *b, int32_t steps_per_candidate);
+int cglbm_batch_sync(cglbm_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,9 @@ EXPORTS = (
     "cglbm_domain_step_count", "cglbm_domain_stream",
     "cglbm_init_bubble", "cglbm_init_jet", "cglbm_init_rti", "cglbm_step", "cglbm_time_steps", "cglbm_sync",
     "cglbm_copy_out", "cglbm_copy_in", "cglbm_reduce", "cglbm_write_vti",
+    "cglbm_batch_create", "cglbm_batch_destroy", "cglbm_batch_add", "cglbm_batch_remove", "cglbm_batch_size",
+    "cglbm_batch_set_tile", "cglbm_batch_set_graph", "cglbm_batch_step", "cglbm_batch_time_steps",
+    "cglbm_batch_time_sequential", "cglbm_batch_autotune", "cglbm_batch_sync",
 )
 
 
@@ -152,6 +155,18 @@ def lib():
         "cglbm_copy_in": ([D, ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
         "cglbm_reduce": ([D, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "cglbm_write_vti": ([D, ctypes.c_char_p], ctypes.c_int),
+        "cglbm_batch_create": ([ctypes.POINTER(D)], ctypes.c_int),
+        "cglbm_batch_destroy": ([D], ctypes.c_int),
+        "cglbm_batch_add": ([D, D], ctypes.c_int),
+        "cglbm_batch_remove": ([D, D], ctypes.c_int),
+        "cglbm_batch_size": ([D], ctypes.c_int32),
+        "cglbm_batch_set_tile": ([D, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
+        "cglbm_batch_set_graph": ([D, ctypes.c_int32], ctypes.c_int),
+        "cglbm_batch_step": ([D, ctypes.c_int32], ctypes.c_int),
+        "cglbm_batch_time_steps": ([D, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "cglbm_batch_time_sequential": ([D, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "cglbm_batch_autotune": ([D, ctypes.c_int32], ctypes.c_int),
+        "cglbm_batch_sync": ([D], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -297,3 +312,67 @@ class Domain:
 
     def write_vti(self, path: str):
         _check(lib().cglbm_write_vti(self._h, path.encode()), "cglbm_write_vti")
+
+
+class Batch:
+    """Many domains stepped together (one launch per kernel class per step)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().cglbm_batch_create(ctypes.byref(h)), "cglbm_batch_create")
+        self._h = h
+        self.members = []
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().cglbm_batch_destroy(self._h)
+            self._h = None
+            self.members = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, dom: "Domain"):
+        _check(lib().cglbm_batch_add(self._h, dom._h), "cglbm_batch_add")
+        self.members.append(dom)
+
+    def remove(self, dom: "Domain"):
+        _check(lib().cglbm_batch_remove(self._h, dom._h), "cglbm_batch_remove")
+        self.members.remove(dom)
+
+    def __len__(self):
+        return int(lib().cglbm_batch_size(self._h))
+
+    def set_tile(self, ty: int, L: int = 16, nbuf: int = 2):
+        _check(lib().cglbm_batch_set_tile(self._h, ty, L, nbuf), "cglbm_batch_set_tile")
+
+    def set_graph(self, bundle: int):
+        _check(lib().cglbm_batch_set_graph(self._h, bundle), "cglbm_batch_set_graph")
+
+    def step(self, nsteps: int = 1):
+        _check(lib().cglbm_batch_step(self._h, nsteps), "cglbm_batch_step")
+
+    def time_steps(self, nsteps: int) -> float:
+        ms = ctypes.c_double()
+        _check(lib().cglbm_batch_time_steps(self._h, nsteps, ctypes.byref(ms)), "cglbm_batch_time_steps")
+        return ms.value
+
+    def time_sequential(self, nsteps: int) -> float:
+        ms = ctypes.c_double()
+        _check(lib().cglbm_batch_time_sequential(self._h, nsteps, ctypes.byref(ms)), "cglbm_batch_time_sequential")
+        return ms.value
+
+    def autotune(self, steps_per_candidate: int = 2):
+        _check(lib().cglbm_batch_autotune(self._h, steps_per_candidate), "cglbm_batch_autotune")
+
+    def sync(self):
+        _check(lib().cglbm_batch_sync(self._h), "cglbm_batch_sync")

@@ -17,26 +17,9 @@
 #include "kernels.hpp"
 #include "lattice.hpp"
 #include "state.hpp"
+#include "domain_internal.hpp"
 
 using namespace cglbm;
-
-struct cglbm_domain {
-    cglbm_config cfg{};
-    int storage = CGLBM_STORE_F32;
-    int kernel = CGLBM_KERNEL_AUTO;
-    int device = 0;
-    int64_t fstride = 0;
-    float *buf[2] = {nullptr, nullptr};
-    cglbm_moments view[2]{};
-    int cur = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double *d_partials = nullptr;
-    double *d_result = nullptr;
-    int64_t steps = 0;
-    cglbm_layout layout{};
-    MarchShape shape{0, 0, 0};  // ty == 0: tuned record or default
-};
 
 namespace {
 
@@ -93,6 +76,12 @@ int check_dom(const cglbm_domain *d) {
     return CGLBM_OK;
 }
 
+// A batch member is stepped on the batch's stream; synchronous member operations wait for it.
+int wait_batch(cglbm_domain *d) {
+    if (d->batch_stream) HTRY(hipStreamSynchronize(d->batch_stream));
+    return CGLBM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -146,6 +135,7 @@ int cglbm_domain_create(const cglbm_config *cfg, int32_t storage, cglbm_domain *
 
 int cglbm_domain_destroy(cglbm_domain *dom) {
     if (!dom) return CGLBM_OK;
+    if (dom->batch) return set_error(CGLBM_ESTATE, "remove the domain from its batch first");
     if (dom->stream) (void)hipStreamSynchronize(dom->stream);
     release(dom);
     return CGLBM_OK;
@@ -192,6 +182,7 @@ int cglbm_domain_set_tile(cglbm_domain *dom, int32_t ty, int32_t L, int32_t nbuf
 int cglbm_domain_autotune(cglbm_domain *dom, int32_t steps_per_candidate, int32_t *ty_out, int32_t *L_out,
                           int32_t *nbuf_out, double *ms_out) {
     if (int rc = check_dom(dom)) return rc;
+    if (int rc = wait_batch(dom)) return rc;
     if (steps_per_candidate < 1 || steps_per_candidate > 1000) return set_error(CGLBM_EINVAL, "bad step count");
     const int tys[3] = {4, 8, 16};
     const int Ls[9] = {2, 4, 6, 8, 12, 16, 24, 32, 64};
@@ -238,6 +229,7 @@ void *cglbm_domain_stream(cglbm_domain *dom) { return dom ? static_cast<void *>(
 int cglbm_init_bubble(cglbm_domain *dom, double x0, double y0, double z0, double radius, double width,
                       int32_t sharp) {
     if (int rc = check_dom(dom)) return rc;
+    if (int rc = wait_batch(dom)) return rc;
     if (!(radius > 0) || (!sharp && !(width > 0))) return set_error(CGLBM_EINVAL, "radius/width must be positive");
     const cglbm_config &c = dom->cfg;
     const int nyt = stored_ny(c);
@@ -264,6 +256,7 @@ int cglbm_init_bubble(cglbm_domain *dom, double x0, double y0, double z0, double
                 }
             }
     dom->steps = 0;
+    if (dom->batch) batch_member_changed(dom->batch);
     if (int rc = upload(dom, h)) return rc;
     return apply_ybc_inplace(dom);
 }
@@ -271,11 +264,13 @@ int cglbm_init_bubble(cglbm_domain *dom, double x0, double y0, double z0, double
 // launch_InitJet (launch.cuh:52-72) followed by main.cu:26's boundary call with B == Aold.
 int cglbm_init_jet(cglbm_domain *dom) {
     if (int rc = check_dom(dom)) return rc;
+    if (int rc = wait_batch(dom)) return rc;
     if (dom->cfg.ybc != CGLBM_Y_JET) return set_error(CGLBM_EINVAL, "jet initialisation needs ybc = CGLBM_Y_JET");
     const int64_t n = stored_cells(dom->cfg);
     std::vector<float> h(11 * n, 0.0f);
     for (int64_t i = 0; i < n; ++i) h[i] = 1.0f;  // red at rest (jetIn.cuh:16-32)
     dom->steps = 0;
+    if (dom->batch) batch_member_changed(dom->batch);
     if (int rc = upload(dom, h)) return rc;
     return apply_ybc_inplace(dom);
 }
@@ -285,6 +280,7 @@ int cglbm_init_jet(cglbm_domain *dom) {
 int cglbm_init_rti(cglbm_domain *dom, double amplitude, double width, int32_t n_random_modes,
                    double random_amplitude, uint32_t seed) {
     if (int rc = check_dom(dom)) return rc;
+    if (int rc = wait_batch(dom)) return rc;
     if (!(width > 0) || n_random_modes < 0 || n_random_modes > 64) return set_error(CGLBM_EINVAL, "bad RTI parameters");
     const cglbm_config &c = dom->cfg;
     const int nyt = stored_ny(c);
@@ -319,6 +315,7 @@ int cglbm_init_rti(cglbm_domain *dom, double amplitude, double width, int32_t n_
             }
         }
     dom->steps = 0;
+    if (dom->batch) batch_member_changed(dom->batch);
     if (int rc = upload(dom, h)) return rc;
     if (c.ybc == CGLBM_Y_WALLS) return apply_ybc_inplace(dom);
     return CGLBM_OK;
@@ -327,6 +324,7 @@ int cglbm_init_rti(cglbm_domain *dom, double amplitude, double width, int32_t n_
 // main.cu:38-55: update, boundary planes, swap.
 int cglbm_step(cglbm_domain *dom, int32_t nsteps) {
     if (int rc = check_dom(dom)) return rc;
+    if (dom->batch) return set_error(CGLBM_ESTATE, "domain is a batch member; step it through its batch");
     if (nsteps < 0) return set_error(CGLBM_EINVAL, "negative step count");
     for (int k = 0; k < nsteps; ++k) {
         const int nxt = 1 - dom->cur;
@@ -354,6 +352,7 @@ int cglbm_time_steps(cglbm_domain *dom, int32_t nsteps, double *ms_total) {
 
 int cglbm_sync(cglbm_domain *dom) {
     if (int rc = check_dom(dom)) return rc;
+    if (int rc = wait_batch(dom)) return rc;
     HTRY(hipStreamSynchronize(dom->stream));
     return CGLBM_OK;
 }
@@ -361,6 +360,7 @@ int cglbm_sync(cglbm_domain *dom) {
 // copy_out_D2H (memory.cuh:209-217), any subset of the 11 fields.
 int cglbm_copy_out(cglbm_domain *dom, uint32_t field_mask, float *host) {
     if (int rc = check_dom(dom)) return rc;
+    if (int rc = wait_batch(dom)) return rc;
     if (!host || (field_mask & ~0x7FFu)) return set_error(CGLBM_EINVAL, "bad arguments");
     const int64_t n = stored_cells(dom->cfg);
     float *const *v = reinterpret_cast<float *const *>(&dom->view[dom->cur]);
@@ -374,6 +374,7 @@ int cglbm_copy_out(cglbm_domain *dom, uint32_t field_mask, float *host) {
 
 int cglbm_copy_in(cglbm_domain *dom, uint32_t field_mask, const float *host) {
     if (int rc = check_dom(dom)) return rc;
+    if (int rc = wait_batch(dom)) return rc;
     if (!host || (field_mask & ~0x7FFu)) return set_error(CGLBM_EINVAL, "bad arguments");
     const int64_t n = stored_cells(dom->cfg);
     float *const *v = reinterpret_cast<float *const *>(&dom->view[dom->cur]);
@@ -387,6 +388,7 @@ int cglbm_copy_in(cglbm_domain *dom, uint32_t field_mask, const float *host) {
 
 int cglbm_reduce(cglbm_domain *dom, double out[6]) {
     if (int rc = check_dom(dom)) return rc;
+    if (int rc = wait_batch(dom)) return rc;
     if (!out) return set_error(CGLBM_EINVAL, "null output");
     const cglbm_config &c = dom->cfg;
     float *const *v = reinterpret_cast<float *const *>(&dom->view[dom->cur]);

@@ -33,6 +33,31 @@ struct MarchArgs {
     uint32_t seed;
     int64_t step;
 };
+// Batched launch: descriptors of member domains (one kernel class: q, ybc, forced), a list of
+// work items {domain, tile x, tile y, chunk}, and the device step base.
+struct BatchDesc {
+    MarchArgs args;      // geometry and physics of the member (A0/B0/step set per launch)
+    float *buf0;         // the member's two states
+    float *buf1;
+    int cur0;            // index of the current state when the descriptors were built
+    int nchunk;          // z-chunks of this member
+    int64_t step0;       // member steps completed when the descriptors were built
+};
+struct BatchLaunch {
+    const BatchDesc *descs;
+    const int4 *items;
+    int nitems;
+    const int64_t *base;
+    int k;
+    int grid;
+    int ybc, forced;
+    MarchShape shape;
+};
+hipError_t launch_batch_q19(const BatchLaunch &L, hipStream_t s);
+hipError_t launch_batch_q27(const BatchLaunch &L, hipStream_t s);
+// device base += n (one thread), ordered on the stream after a bundle of n steps
+hipError_t launch_advance(int64_t *base, int n, hipStream_t s);
+
 // Fills `m` when the 11 fields of each state sit at one stride (the domain layout, or caller
 // arrays carved from one allocation); false otherwise.
 bool march_args(const StepArgs &a, MarchArgs &m);

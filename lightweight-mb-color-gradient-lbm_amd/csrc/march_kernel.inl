@@ -185,24 +185,18 @@ __device__ __forceinline__ void boundary_planes(const MarchArgs &a, const FieldB
 
 enum : int { kRoleOwn = 0, kRoleRows = 1, kRoleCols = 2, kRoleNone = 3 };
 
+// One work item: tile (tix, tiy) of chunk `chunk` (of nchunk) of the domain described by `a`,
+// executed by the whole work-group. Ends with the work-group synchronised so the caller may run
+// another item with the same LDS.
 template <int Q, int YBC, int TY, int NBUF, bool FORCED>
-__global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
-    march_kernel(MarchArgs a, int ntx, int nty, int nchunk, int nblocks) {
+__device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBUF> &sh, int tix, int tiy, int chunk,
+                                           int nchunk) {
     constexpr int NOWN = TX * TY;
     constexpr int PX = TX + 4, PY = TY + 4, NPSI = PX * PY;
     constexpr int NPASS = (NPSI + 127) / 128;  // psi cells per ring lane
     constexpr uint32_t ALL = pair_mask_all<Q>();
     constexpr uint32_t MX = pair_mask_where<Q>(0), MY = pair_mask_where<Q>(1), MZ = pair_mask_where<Q>(2);
 
-    __shared__ Shared<Q, TY, NBUF> sh;
-
-    const int b = blockIdx.x;
-    const int per = gridDim.x / 8;
-    const int logical = (b % 8) * per + b / 8;  // XCD-contiguous tile ranges
-    if (logical >= nblocks) return;
-    const int tix = logical % ntx;
-    const int tiy = (logical / ntx) % nty;
-    const int chunk = logical / (ntx * nty);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int z0 = static_cast<int>((static_cast<int64_t>(chunk) * nz) / nchunk);
     const int z1 = static_cast<int>((static_cast<int64_t>(chunk + 1) * nz) / nchunk);
@@ -440,6 +434,48 @@ __global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
         }
         if constexpr (NBUF == 1) __syncthreads();
     }
+    if constexpr (NBUF == 2) __syncthreads();
+}
+
+template <int Q, int YBC, int TY, int NBUF, bool FORCED>
+__global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
+    march_kernel(MarchArgs a, int ntx, int nty, int nchunk, int nblocks) {
+    __shared__ Shared<Q, TY, NBUF> sh;
+    const int b = blockIdx.x;
+    const int per = gridDim.x / 8;
+    const int logical = (b % 8) * per + b / 8;  // XCD-contiguous tile ranges
+    if (logical >= nblocks) return;
+    const int tix = logical % ntx;
+    const int tiy = (logical / ntx) % nty;
+    const int chunk = logical / (ntx * nty);
+    march_tile<Q, YBC, TY, NBUF, FORCED>(a, sh, tix, tiy, chunk, nchunk);
+}
+
+// Batched form: many domains of one kernel class in one launch. Each work item names a domain
+// descriptor and a tile/chunk of it; blocks take items in XCD-contiguous ranges and loop while
+// items remain, so any grid size covers the list. The step counter (parity of the ping-pong and
+// the inlet perturbation key) is base[0] + k, base living in device memory so a captured
+// multi-step graph can be replayed.
+template <int Q, int YBC, int TY, int NBUF, bool FORCED>
+__global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
+    march_batch_kernel(const BatchDesc *__restrict__ descs, const int4 *__restrict__ items, int nitems,
+                       const int64_t *__restrict__ base, int k) {
+    __shared__ Shared<Q, TY, NBUF> sh;
+    const int64_t kk = base[0] + k;
+    const int nper = (nitems + 7) / 8;
+    for (int b = blockIdx.x; b < nper * 8; b += gridDim.x) {
+        const int logical = (b % 8) * nper + b / 8;
+        if (logical >= nitems) continue;
+        const int4 it = items[logical];
+        const BatchDesc &d = descs[it.x];
+        MarchArgs a = d.args;
+        const int64_t steps_done = d.step0 + kk;  // member steps completed before this launch
+        const int parity = d.cur0 ^ static_cast<int>(kk & 1);
+        a.A0 = parity ? d.buf1 : d.buf0;
+        a.B0 = parity ? d.buf0 : d.buf1;
+        a.step = steps_done + 1;
+        march_tile<Q, YBC, TY, NBUF, FORCED>(a, sh, it.y, it.z, it.w, d.nchunk);
+    }
 }
 
 template <int Q, int YBC, int TY, int NBUF>
@@ -484,6 +520,36 @@ hipError_t launch_q(const MarchArgs &a, MarchShape m, hipStream_t s) {
 template <int Q>
 hipError_t launch_march_q(const MarchArgs &a, MarchShape m, hipStream_t s) {
     return a.ybc == kYPeriodic ? launch_q<Q, kYPeriodic>(a, m, s) : launch_q<Q, kYJet>(a, m, s);
+}
+
+template <int Q, int YBC, int TY, int NBUF, bool FORCED>
+hipError_t launch_batch_shape(const BatchLaunch &L, hipStream_t s) {
+    march_batch_kernel<Q, YBC, TY, NBUF, FORCED>
+        <<<L.grid, TX * TY + 128, 0, s>>>(L.descs, L.items, L.nitems, L.base, L.k);
+    return hipGetLastError();
+}
+
+template <int Q, int YBC, bool FORCED>
+hipError_t launch_batch_qy(const BatchLaunch &L, hipStream_t s) {
+    const bool one = L.shape.nbuf == 1;
+    switch (L.shape.ty) {
+    case 4:
+        return one ? launch_batch_shape<Q, YBC, 4, 1, FORCED>(L, s) : launch_batch_shape<Q, YBC, 4, 2, FORCED>(L, s);
+    case 16:
+        if constexpr (fits2<Q, 16>()) {
+            if (!one) return launch_batch_shape<Q, YBC, 16, 2, FORCED>(L, s);
+        }
+        return launch_batch_shape<Q, YBC, 16, 1, FORCED>(L, s);
+    default:
+        return one ? launch_batch_shape<Q, YBC, 8, 1, FORCED>(L, s) : launch_batch_shape<Q, YBC, 8, 2, FORCED>(L, s);
+    }
+}
+
+template <int Q>
+hipError_t launch_batch_q(const BatchLaunch &L, hipStream_t s) {
+    if (L.ybc == kYPeriodic) return launch_batch_qy<Q, kYPeriodic, false>(L, s);
+    if (L.forced) return launch_batch_qy<Q, kYJet, true>(L, s);
+    return launch_batch_qy<Q, kYJet, false>(L, s);
 }
 
 }  // namespace march
