@@ -14,6 +14,7 @@
  *   swap_moments (memory.cuh:200-203) .................... inside cglbm_step
  *   copy_out_D2H (memory.cuh:209-217) .................... cglbm_copy_out
  *   write_vti_step_device (io/vtiWriter.cuh:151-164) ..... cglbm_write_vti
+ *   write_vti (io/vtiWriter.cuh:92-145) ................... cglbm_write_vti_host
  *   MlupsCounter (utilities/mlups_count.cuh:14-144) ...... cglbm_time_steps
  *
  * Every entry point returns 0 on success or a negative CGLBM_E* code (no exit(), unlike
@@ -158,8 +159,16 @@ int cglbm_copy_in(cglbm_domain *dom, uint32_t field_mask, const float *host);
  * negative-density cells. Synchronous. */
 int cglbm_reduce(cglbm_domain *dom, double out[6]);
 
-/* ASCII VTI with rhor rhob ux uy uz over the stored extent (io/vtiWriter.cuh:60-145). */
+/* ASCII VTI with rhor rhob ux uy uz of the current state over the stored extent
+ * (write_vti_step_device, io/vtiWriter.cuh:151-164). Synchronous. */
 int cglbm_write_vti(cglbm_domain *dom, const char *path);
+
+/* The reference's writer itself (write_vti(filename, LbmHost), io/vtiWriter.cuh:92-145): five
+ * host arrays of nx*ny*nz floats, x fastest, any extent (the reference passes its full grid,
+ * ghost layers included). Creates missing parent folders. Byte-identical output for the same
+ * arrays. Needs no GPU. CGLBM_EIO where the reference throws std::runtime_error. */
+int cglbm_write_vti_host(const char *path, int nx, int ny, int nz, const float *rhor, const float *rhob,
+                         const float *ux, const float *uy, const float *uz);
 
 /* ---- Batches: many domains stepped together ----------------------------------------------
  * The reference runs one domain per process (main.cu). A batch steps any number of domains with

@@ -27,6 +27,7 @@ FIELDS = ("rhor", "rhob", "ux", "uy", "uz", "Pixx", "Pixy", "Piyy", "Piyz", "Piz
 Y_PERIODIC, Y_JET, Y_WALLS = 0, 1, 2
 OUTLET_CONVECTIVE, OUTLET_ZERO_GRADIENT = 0, 1
 STORE_F32, STORE_F16 = 0, 1
+OK, EINVAL, ENOMEM, EHIP, EIO, ESTATE = 0, -1, -2, -3, -4, -5
 KERNEL_AUTO, KERNEL_GATHER, KERNEL_MARCH = 0, 1, 2
 
 # Every symbol include/cglbm.h declares (checked by the CPU test suite).
@@ -36,7 +37,7 @@ EXPORTS = (
     "cglbm_domain_bytes", "cglbm_domain_set_kernel", "cglbm_domain_set_tile", "cglbm_domain_autotune",
     "cglbm_domain_step_count", "cglbm_domain_stream",
     "cglbm_init_bubble", "cglbm_init_jet", "cglbm_init_rti", "cglbm_step", "cglbm_time_steps", "cglbm_sync",
-    "cglbm_copy_out", "cglbm_copy_in", "cglbm_reduce", "cglbm_write_vti",
+    "cglbm_copy_out", "cglbm_copy_in", "cglbm_reduce", "cglbm_write_vti", "cglbm_write_vti_host",
     "cglbm_batch_create", "cglbm_batch_destroy", "cglbm_batch_add", "cglbm_batch_remove", "cglbm_batch_size",
     "cglbm_batch_set_tile", "cglbm_batch_set_graph", "cglbm_batch_step", "cglbm_batch_time_steps",
     "cglbm_batch_time_sequential", "cglbm_batch_autotune", "cglbm_batch_sync",
@@ -155,6 +156,8 @@ def lib():
         "cglbm_copy_in": ([D, ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
         "cglbm_reduce": ([D, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "cglbm_write_vti": ([D, ctypes.c_char_p], ctypes.c_int),
+        "cglbm_write_vti_host": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5,
+                                 ctypes.c_int),
         "cglbm_batch_create": ([ctypes.POINTER(D)], ctypes.c_int),
         "cglbm_batch_destroy": ([D], ctypes.c_int),
         "cglbm_batch_add": ([D, D], ctypes.c_int),
@@ -204,6 +207,17 @@ def launch_ybc(B: Moments, Aold: Moments, cfg: Config, layout: Layout | None = N
     L = layout if layout is not None else compact_layout(cfg)
     _check(lib().cglbm_launch_ybc(ctypes.byref(B), ctypes.byref(Aold), ctypes.byref(cc), ctypes.byref(L), step,
                                   stream), "cglbm_launch_ybc")
+
+
+def write_vti_host(path: str, fields) -> None:
+    """The reference's ``write_vti(filename, LbmHost)`` (io/vtiWriter.cuh:92-145): ``fields`` holds
+    rhor, rhob, ux, uy, uz as float32 arrays of one shape (nz, ny, nx), x fastest. Needs no GPU."""
+    arrs = [np.ascontiguousarray(f, dtype=np.float32) for f in fields]
+    if len(arrs) != 5 or any(a.ndim != 3 or a.shape != arrs[0].shape for a in arrs):
+        raise ValueError("write_vti_host needs five float32 arrays of one (nz, ny, nx) shape")
+    nz, ny, nx = arrs[0].shape
+    _check(lib().cglbm_write_vti_host(path.encode(), nx, ny, nz, *[a.ctypes.data for a in arrs]),
+           "cglbm_write_vti_host")
 
 
 def domain_bytes(cfg: Config, storage: int = STORE_F32) -> int:

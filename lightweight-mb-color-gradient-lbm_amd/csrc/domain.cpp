@@ -17,6 +17,7 @@
 #include "kernels.hpp"
 #include "lattice.hpp"
 #include "state.hpp"
+#include "vti.hpp"
 #include "domain_internal.hpp"
 
 using namespace cglbm;
@@ -399,45 +400,17 @@ int cglbm_reduce(cglbm_domain *dom, double out[6]) {
     return CGLBM_OK;
 }
 
-// write_vti (io/vtiWriter.cuh:92-145): ASCII ImageData, Float32 point arrays rhor rhob ux uy uz,
-// x fastest, one row per (z, y), "%.8e "-style values.
+// write_vti_step_device (io/vtiWriter.cuh:151-164): copy rhor rhob ux uy uz of the current state
+// to the host and print them over the stored extent with the reference's text layout.
 int cglbm_write_vti(cglbm_domain *dom, const char *path) {
     if (int rc = check_dom(dom)) return rc;
     if (!path) return set_error(CGLBM_EINVAL, "null path");
     const cglbm_config &c = dom->cfg;
-    const int nyt = stored_ny(c);
     const int64_t n = stored_cells(c);
     std::vector<float> h(5 * n);
     if (int rc = cglbm_copy_out(dom, 0x1Fu, h.data())) return rc;
-    std::ofstream out(path);
-    if (!out) return set_error(CGLBM_EIO, "cannot open VTI file for writing");
-    out << "<?xml version=\"1.0\"?>\n";
-    out << "<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
-    out << "  <ImageData WholeExtent=\"0 " << c.nx - 1 << " 0 " << nyt - 1 << " 0 " << c.nz - 1
-        << "\" Origin=\"0 0 0\" Spacing=\"1 1 1\">\n";
-    out << "    <Piece Extent=\"0 " << c.nx - 1 << " 0 " << nyt - 1 << " 0 " << c.nz - 1 << "\">\n";
-    out << "      <PointData Scalars=\"rhor\">\n";
-    static const char *names[5] = {"rhor", "rhob", "ux", "uy", "uz"};
-    out << std::scientific << std::setprecision(8);
-    for (int f = 0; f < 5; ++f) {
-        out << "        <DataArray type=\"Float32\" Name=\"" << names[f] << "\" format=\"ascii\">\n";
-        const float *p = h.data() + f * n;
-        for (int z = 0; z < c.nz; ++z)
-            for (int y = 0; y < nyt; ++y) {
-                const float *row = p + static_cast<int64_t>(c.nx) * (y + static_cast<int64_t>(nyt) * z);
-                for (int x = 0; x < c.nx; ++x) out << row[x] << " ";
-                out << "\n";
-            }
-        out << "        </DataArray>\n";
-    }
-    out << "      </PointData>\n";
-    out << "      <CellData>\n";
-    out << "      </CellData>\n";
-    out << "    </Piece>\n";
-    out << "  </ImageData>\n";
-    out << "</VTKFile>\n";
-    if (!out) return set_error(CGLBM_EIO, "error while writing the VTI file");
-    return CGLBM_OK;
+    const float *const f[5] = {h.data(), h.data() + n, h.data() + 2 * n, h.data() + 3 * n, h.data() + 4 * n};
+    return write_vti_arrays(path, c.nx, stored_ny(c), c.nz, f);
 }
 
 }  // extern "C"

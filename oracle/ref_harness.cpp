@@ -9,6 +9,7 @@
 //   init_jet_bulk / init_jet_inlet      jet/jetIn.cuh:16-60
 //   init_static_bubble                  bubble/bubbleIn.cuh:36-52
 //   init_rti_domain                     rti/rtiIn.cuh:47-71
+//   write_vti                           io/vtiWriter.cuh:92-145 (REF_VTI snapshots only)
 // in the step order of main.cu:38-55 (RCS over interior cells, then the jet boundary kernel
 // with (B, Aold), then the pointer swap), with the interior guards of kernels.cu:22-131.
 //
@@ -46,6 +47,7 @@ inline label_t ref_force_y_periodic(const int y) noexcept { return wrapy(static_
 #include "jet/boundary_conditions.cuh"
 #include "bubble/bubbleIn.cuh"
 #include "rti/rtiIn.cuh"
+#include "io/vtiWriter.cuh"
 
 #include <chrono>
 #include <cmath>
@@ -80,7 +82,25 @@ bool g_periodic_y =
 label_t y_lo() { return g_periodic_y ? 1u : 0u; }
 label_t y_hi() { return g_periodic_y ? NY - 2u : NY - 1u; }
 
+// With REF_VTI set in the environment, every snapshot is also written by the reference's own VTI
+// writer (io/vtiWriter.cuh:92-145, full extent incl. ghosts) to <path>.vti, and the five fields it
+// prints are dumped over the full extent to <path>.full5.
+void dump_vti(const HostState &s, const std::string &path) {
+    LbmHost h{};
+    h.rhor = const_cast<float *>(s.buf[0].data());
+    h.rhob = const_cast<float *>(s.buf[1].data());
+    h.ux = const_cast<float *>(s.buf[2].data());
+    h.uy = const_cast<float *>(s.buf[3].data());
+    h.uz = const_cast<float *>(s.buf[4].data());
+    write_vti(std::filesystem::path(path + ".vti"), h);
+    FILE *f = std::fopen((path + ".full5").c_str(), "wb");
+    if (!f) { std::perror(path.c_str()); std::exit(2); }
+    for (int fi = 0; fi < 5; ++fi) std::fwrite(s.buf[fi].data(), sizeof(float), Ncells, f);
+    std::fclose(f);
+}
+
 void dump(const HostState &s, const std::string &path) {
+    if (std::getenv("REF_VTI")) dump_vti(s, path);
     FILE *f = std::fopen(path.c_str(), "wb");
     if (!f) { std::perror(path.c_str()); std::exit(2); }
     std::vector<float> row;

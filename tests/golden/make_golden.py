@@ -11,6 +11,10 @@ each case, and stores compact fixtures:
 * ``idx``        fixed sample of flat cell indices; ``val_<k>`` the 11 fields at those cells;
 * ``full_<k>``   the whole state, for the small random-moment cases (their input included).
 
+VTI fixtures (``vti_<case>.npz``): the reference's own writer (io/vtiWriter.cuh:92-145, compiled into
+the harness) prints a snapshot over its full grid, ghost layers included. Stored: the five printed
+fields over that full extent (``full5``), the sha256 and byte length of the file it wrote.
+
 Usage: python tests/golden/make_golden.py [--ref /root/reference]
 """
 from __future__ import annotations
@@ -33,7 +37,7 @@ REF_FILES = [
     "lbm.cuh", "pdf.cuh", "forces.cuh", "collisionOperators.cuh", "stencil_ct.cuh", "constants.cuh",
     "memory.cuh", "utilities/bounds.cuh", "utilities/indexing.cuh", "utilities/mathUtilities.cuh",
     "utilities/constexprFor.cuh", "utilities/types.cuh", "jet/boundary_conditions.cuh", "jet/jetIn.cuh",
-    "bubble/bubbleIn.cuh", "rti/rtiIn.cuh",
+    "bubble/bubbleIn.cuh", "rti/rtiIn.cuh", "io/vtiWriter.cuh",
 ]
 
 SMALL = dict(tau=0.8, sigma=0.01, beta=0.7, jet_velocity=0.05)
@@ -67,6 +71,15 @@ CASES = [
      dict(nx=24, ny=24, nz=24, q=19, ybc=0, **SMALL), False),
     ("rti_init", "ref_rti_init", "rti_init", 0, [],
      dict(nx=32, ny=64, nz=32, q=19, ybc=2, **SMALL), True),
+]
+
+
+# name, binary, mode, arg, step, oracle params
+VTI_CASES = [
+    ("vti_rand_q19", "ref_rand_q19", "random", 1, 2,
+     dict(nx=10, ny=12, nz=8, q=19, ybc=1, jet_radius=3.0, **SMALL)),
+    ("vti_jet_small_q19", "ref_jet_small_q19", "jet", 0, 10,
+     dict(nx=16, ny=32, nz=16, q=19, ybc=1, jet_radius=4.0, **SMALL)),
 ]
 
 
@@ -110,6 +123,23 @@ def main() -> int:
                         reference_sha256=ref_hash)
             arrays["meta"] = np.array(json.dumps(meta))
             np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **arrays)
+            print("wrote", name, file=sys.stderr)
+        for name, binary, mode, arg, step, params in VTI_CASES:
+            if args.only and name != args.only:
+                continue
+            prefix = os.path.join(tmp, name)
+            cmd = [os.path.join(ORACLE, "_ref", binary), mode, str(arg), str(step), prefix]
+            subprocess.run(cmd, check=True, env=dict(os.environ, REF_VTI="1"), stdout=subprocess.DEVNULL)
+            full = (params["nx"] + 2, params["ny"] + 2, params["nz"] + 2)
+            with open(f"{prefix}_step{step}.bin.vti", "rb") as fh:
+                text = fh.read()
+            f5 = np.fromfile(f"{prefix}_step{step}.bin.full5", dtype=np.float32).reshape(
+                5, full[2], full[1], full[0])
+            meta = dict(name=name, binary=binary, mode=mode, arg=arg, step=step, params=params,
+                        reference_sha256=ref_hash)
+            np.savez_compressed(os.path.join(OUT, f"{name}.npz"), full5=f5,
+                                vti_sha256=np.array(hashlib.sha256(text).hexdigest()),
+                                vti_bytes=np.array(len(text)), meta=np.array(json.dumps(meta)))
             print("wrote", name, file=sys.stderr)
     return 0
 
