@@ -27,6 +27,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE_F32 = 88
+# FP16-deviation storage: rhor, rhob binary32 (2 x 4 B), nine binary16 deviations (9 x 2 B), read + write
+BYTES_PER_UPDATE_F16 = 52
 
 
 def _load():
@@ -34,9 +36,10 @@ def _load():
     return ge._import_package()
 
 
-def measure(cg, q: int, n: int, steps: int, warmup: int, kernel: int, tile=None):
+def measure(cg, q: int, n: int, steps: int, warmup: int, kernel: int, tile=None, storage=None):
     cfg = cg.Config(nx=n, ny=n, nz=n, q=q, tau=0.8, sigma=0.01, beta=0.7)
-    with cg.Domain(cfg, kernel=kernel) as dom:
+    storage = cg.STORE_F32 if storage is None else storage
+    with cg.Domain(cfg, storage=storage, kernel=kernel) as dom:
         dom.init_bubble(radius=24.0 * n / 128.0, width=4.0)
         m0 = dom.reduce()
         shape = None
@@ -53,7 +56,8 @@ def measure(cg, q: int, n: int, steps: int, warmup: int, kernel: int, tile=None)
     cells = cfg.interior_cells
     sec = ms / 1e3
     mlups = cells * steps / sec / 1e6
-    achieved = BYTES_PER_UPDATE_F32 * cells / (sec / steps) / 1e9
+    per_update = BYTES_PER_UPDATE_F16 if storage == cg.STORE_F16 else BYTES_PER_UPDATE_F32
+    achieved = per_update * cells / (sec / steps) / 1e9
     return dict(
         mlups=mlups,
         ms_per_step=ms / steps,
@@ -208,6 +212,9 @@ def main() -> int:
     print(f"bench: D3Q19 {r19['mlups']:.0f} MLUPS; D3Q27 next", file=sys.stderr, flush=True)
     r27 = measure(cg, 27, args.n, args.steps, args.warmup, kern, t27)
     print(f"bench: D3Q27 {r27['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    print("bench: D3Q19 FP16-deviation storage", file=sys.stderr, flush=True)
+    r16 = measure(cg, 19, args.n, args.steps, args.warmup, cg.KERNEL_AUTO, None, storage=cg.STORE_F16)
+    print(f"bench: D3Q19 FP16 storage {r16['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
     batched = None
     if not args.no_batched:
         print("bench: 256 batched domains", file=sys.stderr, flush=True)
@@ -244,7 +251,13 @@ def main() -> int:
         "d3q27": {"value": round(r27["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(r27["ms_per_step"], 5),
                   "roofline": {"bound": "hbm", "achieved": round(r27["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": round(r27["achieved_gbs"] / HBM_PEAK_GBS, 4)}},
-        "mass_drift_rel": {"d3q19": r19["mass_drift_rel"], "d3q27": r27["mass_drift_rel"]},
+        "d3q19_f16_storage": {"value": round(r16["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(r16["ms_per_step"], 5),
+                              "bytes_per_update": BYTES_PER_UPDATE_F16, "tile": r16["tile"],
+                              "roofline": {"bound": "hbm", "achieved": round(r16["achieved_gbs"], 1),
+                                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                           "frac": round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4)}},
+        "mass_drift_rel": {"d3q19": r19["mass_drift_rel"], "d3q27": r27["mass_drift_rel"],
+                           "d3q19_f16_storage": r16["mass_drift_rel"]},
         "batched": batched,
     }
     print(json.dumps(line))

@@ -20,7 +20,7 @@ using namespace cglbm;
 namespace {
 
 struct Class {
-    int q, ybc, forced;
+    int q, ybc, forced, store;
     std::vector<cglbm_domain *> members;
     MarchShape shape{8, 16, 2};
     bool shape_set = false;
@@ -82,15 +82,17 @@ int rebuild(cglbm_batch *b) {
     for (cglbm_domain *d : b->members) {
         const int forced = d->cfg.gravity != 0.0f ? 1 : 0;
         auto it = std::find_if(b->classes.begin(), b->classes.end(), [&](const Class &c) {
-            return c.q == d->cfg.q && c.ybc == (d->cfg.ybc == CGLBM_Y_PERIODIC ? 0 : 1) && c.forced == forced;
+            return c.q == d->cfg.q && c.ybc == (d->cfg.ybc == CGLBM_Y_PERIODIC ? 0 : 1) && c.forced == forced &&
+                   c.store == d->storage;
         });
         if (it == b->classes.end()) {
             Class c;
             c.q = d->cfg.q;
             c.ybc = d->cfg.ybc == CGLBM_Y_PERIODIC ? 0 : 1;
             c.forced = forced;
+            c.store = d->storage;
             for (const Class &o : old)
-                if (o.q == c.q && o.ybc == c.ybc && o.forced == c.forced && o.shape_set) {
+                if (o.q == c.q && o.ybc == c.ybc && o.forced == c.forced && o.store == c.store && o.shape_set) {
                     c.shape = o.shape;
                     c.shape_set = true;
                 }
@@ -108,7 +110,9 @@ int rebuild(cglbm_batch *b) {
         for (cglbm_domain *d : c.members) {
             StepArgs sa = make_step_args(&d->view[0], &d->view[1], &d->cfg, &d->layout, 0);
             BatchDesc bd{};
-            if (!march_args(sa, bd.args)) return set_error(CGLBM_EINVAL, "batch member not addressable by the fast path");
+            const bool ok = d->storage == CGLBM_STORE_F16 ? march_args_f16(sa, d->buf[0], d->buf[1], d->fstride, bd.args)
+                                                          : march_args(sa, bd.args);
+            if (!ok) return set_error(CGLBM_EINVAL, "batch member not addressable by the fast path");
             bd.buf0 = d->buf[0];
             bd.buf1 = d->buf[1];
             bd.cur0 = d->cur;
@@ -166,9 +170,9 @@ int enqueue_step(cglbm_batch *b, int k) {
         L.grid = b->grid_per_class > 0 ? std::min(b->grid_per_class, (c.nitems + 7) / 8 * 8) : (c.nitems + 7) / 8 * 8;
         L.ybc = c.ybc;
         L.forced = c.forced;
+        L.store = c.store;
         L.shape = class_shape(b, c);
-        // item descriptors index within the class
-        const hipError_t e = c.q == 19 ? launch_batch_q19(L, b->stream) : launch_batch_q27(L, b->stream);
+        const hipError_t e = launch_batch(L, c.q, b->stream);
         if (e != hipSuccess) return herr(e, "batch launch");
     }
     return CGLBM_OK;
@@ -257,7 +261,6 @@ int cglbm_batch_add(cglbm_batch *b, cglbm_domain *d) {
     if (int rc = check_batch(b)) return rc;
     if (!d) return set_error(CGLBM_EINVAL, "null domain");
     if (d->batch) return set_error(CGLBM_ESTATE, "domain already belongs to a batch");
-    if (d->storage != CGLBM_STORE_F32) return set_error(CGLBM_EINVAL, "batch members use FP32 storage");
     if (d->cfg.gravity != 0.0f && d->cfg.ybc == CGLBM_Y_PERIODIC)
         return set_error(CGLBM_EINVAL, "a forced periodic domain has no batched kernel");
     HTRY(hipStreamSynchronize(d->stream));
@@ -354,7 +357,7 @@ int cglbm_batch_time_sequential(cglbm_batch *b, int32_t nsteps, double *ms_total
         for (cglbm_domain *d : b->members) {
             const int nxt = 1 - d->cur;
             if (int rc = launch_step(&d->view[d->cur], &d->view[nxt], &d->cfg, &d->layout, d->steps + 1, d->kernel,
-                                     &d->shape, b->stream))
+                                     &d->shape, b->stream, d->storage, d->fstride))
                 return rc;
             d->cur = nxt;
             ++d->steps;
@@ -406,8 +409,9 @@ int cglbm_batch_autotune(cglbm_batch *b, int32_t steps_per_candidate) {
                     Lh.grid = (c.nitems + 7) / 8 * 8;
                     Lh.ybc = c.ybc;
                     Lh.forced = c.forced;
+                    Lh.store = c.store;
                     Lh.shape = c.shape;
-                    auto go = [&]() { return c.q == 19 ? launch_batch_q19(Lh, b->stream) : launch_batch_q27(Lh, b->stream); };
+                    auto go = [&]() { return launch_batch(Lh, c.q, b->stream); };
                     HTRY(go());
                     HTRY(hipEventRecord(b->ev0, b->stream));
                     for (int k = 0; k < steps_per_candidate; ++k) HTRY(go());

@@ -17,6 +17,7 @@
 #include "kernels.hpp"
 #include "lattice.hpp"
 #include "state.hpp"
+#include "storage.hpp"
 #include "vti.hpp"
 #include "domain_internal.hpp"
 
@@ -35,11 +36,31 @@ int herr(hipError_t e, const char *what) {
         if (e_ != hipSuccess) return herr(e_, #expr); \
     } while (0)
 
+// FP32: the 11 field pointers of each state. FP16: only rhor/rhob are FP32 arrays; the other
+// pointers stay null (the step takes the state base, the copies go through an FP32 staging area).
 void make_views(cglbm_domain *d) {
     for (int b = 0; b < 2; ++b) {
         float **v = reinterpret_cast<float **>(&d->view[b]);
-        for (int f = 0; f < 11; ++f) v[f] = d->buf[b] + f * d->fstride;
+        for (int f = 0; f < 11; ++f)
+            v[f] = (d->storage == CGLBM_STORE_F32 || f < 2) ? d->buf[b] + f * d->fstride : nullptr;
     }
+}
+
+// Transient FP32 image of a state for an FP16 domain (11 x fstride floats, compact layout), with
+// its field views; freed by the destructor. Only the host copies and initialisers use it.
+struct Staging {
+    float *p = nullptr;
+    cglbm_moments view{};
+    ~Staging() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+int make_staging(const cglbm_domain *d, Staging &st) {
+    HTRY(hipMalloc(&st.p, sizeof(float) * 11 * static_cast<size_t>(d->fstride)));
+    float **v = reinterpret_cast<float **>(&st.view);
+    for (int f = 0; f < 11; ++f) v[f] = st.p + f * d->fstride;
+    return CGLBM_OK;
 }
 
 void release(cglbm_domain *d) {
@@ -54,20 +75,29 @@ void release(cglbm_domain *d) {
     delete d;
 }
 
-// Host staging of the 11 current fields (compact layout).
-int upload(cglbm_domain *d, const std::vector<float> &host) {
-    const int64_t n = stored_cells(d->cfg);
-    float *const *v = reinterpret_cast<float *const *>(&d->view[d->cur]);
-    for (int f = 0; f < 11; ++f)
-        HTRY(hipMemcpyAsync(v[f], host.data() + f * n, sizeof(float) * n, hipMemcpyHostToDevice, d->stream));
-    HTRY(hipStreamSynchronize(d->stream));
+int apply_ybc_views(cglbm_domain *d, const cglbm_moments &m) {
+    BcArgs b = make_bc_args(&m, &m, &d->cfg, &d->layout, 0);
+    HTRY(launch_ybc(b, d->stream));
     return CGLBM_OK;
 }
 
-int apply_ybc_inplace(cglbm_domain *d) {
-    if (d->cfg.ybc == CGLBM_Y_PERIODIC) return CGLBM_OK;
-    BcArgs b = make_bc_args(&d->view[d->cur], &d->view[d->cur], &d->cfg, &d->layout, 0);
-    HTRY(launch_ybc(b, d->stream));
+// Initial state from the host (11 fields, compact layout), then, if `bc`, the boundary planes
+// computed from it in place (main.cu:26 calls the boundary kernel with B == Aold). FP16 domains
+// assemble the FP32 state in staging and encode it once at the end.
+int upload_init(cglbm_domain *d, const std::vector<float> &host, bool bc) {
+    const int64_t n = stored_cells(d->cfg);
+    Staging st;
+    const cglbm_moments *m = &d->view[d->cur];
+    if (d->storage == CGLBM_STORE_F16) {
+        if (int rc = make_staging(d, st)) return rc;
+        m = &st.view;
+    }
+    float *const *v = reinterpret_cast<float *const *>(m);
+    for (int f = 0; f < 11; ++f)
+        HTRY(hipMemcpyAsync(v[f], host.data() + f * n, sizeof(float) * n, hipMemcpyHostToDevice, d->stream));
+    if (bc && d->cfg.ybc != CGLBM_Y_PERIODIC)
+        if (int rc = apply_ybc_views(d, *m)) return rc;
+    if (d->storage == CGLBM_STORE_F16) HTRY(launch_pack_f16(st.p, d->buf[d->cur], d->fstride, n, d->stream));
     HTRY(hipStreamSynchronize(d->stream));
     return CGLBM_OK;
 }
@@ -89,9 +119,8 @@ extern "C" {
 
 int64_t cglbm_domain_bytes(const cglbm_config *cfg, int32_t storage) {
     if (validate_config(cfg)) return -1;
-    (void)storage;
-    return 2 * 11 * field_stride(*cfg) * static_cast<int64_t>(sizeof(float)) +
-           static_cast<int64_t>(sizeof(double)) * (kReduceBlocks * 6 + 8);
+    if (storage != CGLBM_STORE_F32 && storage != CGLBM_STORE_F16) return -1;
+    return 2 * state_bytes(storage, field_stride(*cfg)) + static_cast<int64_t>(sizeof(double)) * (kReduceBlocks * 6 + 8);
 }
 
 // allocate_device_memory (memory.cuh:153-164): two zeroed 11-field states.
@@ -99,7 +128,15 @@ int cglbm_domain_create(const cglbm_config *cfg, int32_t storage, cglbm_domain *
     if (!out) return set_error(CGLBM_EINVAL, "null output");
     *out = nullptr;
     if (int rc = validate_config(cfg)) return rc;
-    if (storage != CGLBM_STORE_F32) return set_error(CGLBM_EINVAL, "storage mode not available for single domains");
+    if (storage != CGLBM_STORE_F32 && storage != CGLBM_STORE_F16) return set_error(CGLBM_EINVAL, "unknown storage mode");
+    if (storage == CGLBM_STORE_F16) {
+        if (cfg->gravity != 0.0f && cfg->ybc == CGLBM_Y_PERIODIC)
+            return set_error(CGLBM_EINVAL, "FP16 storage: a body force needs bounded y (walls or jet planes)");
+        if (cfg->nx < 3 || cfg->ny < 3 || cfg->nz < 3)
+            return set_error(CGLBM_EINVAL, "FP16 storage needs at least 3 cells along each axis");
+        if (state_bytes(kStoreF16, field_stride(*cfg)) >= (int64_t(1) << 32))
+            return set_error(CGLBM_EINVAL, "FP16 storage: a state must stay below 4 GiB");
+    }
     cglbm_domain *d = new (std::nothrow) cglbm_domain;
     if (!d) return set_error(CGLBM_ENOMEM, "host allocation failed");
     d->cfg = *cfg;
@@ -109,7 +146,7 @@ int cglbm_domain_create(const cglbm_config *cfg, int32_t storage, cglbm_domain *
         release(d);
         return set_error(CGLBM_EHIP, "no HIP device");
     }
-    const size_t bytes = sizeof(float) * 11 * static_cast<size_t>(d->fstride);
+    const size_t bytes = static_cast<size_t>(state_bytes(storage, d->fstride));
     for (int b = 0; b < 2; ++b) {
         if (hipMalloc(&d->buf[b], bytes) != hipSuccess) {
             release(d);
@@ -145,6 +182,8 @@ int cglbm_domain_destroy(cglbm_domain *dom) {
 int cglbm_domain_moments(cglbm_domain *dom, int32_t which, cglbm_moments *out) {
     if (int rc = check_dom(dom)) return rc;
     if (!out || (which != 0 && which != 1)) return set_error(CGLBM_EINVAL, "bad arguments");
+    if (dom->storage != CGLBM_STORE_F32)
+        return set_error(CGLBM_EINVAL, "FP16-deviation states have no FP32 field views (use cglbm_copy_out)");
     *out = dom->view[which == 0 ? dom->cur : 1 - dom->cur];
     return CGLBM_OK;
 }
@@ -197,12 +236,13 @@ int cglbm_domain_autotune(cglbm_domain *dom, int32_t steps_per_candidate, int32_
             const MarchShape m{ty, L, nbuf};
             // warm-up launch, then the timed ones
             if (int rc = launch_step(&dom->view[dom->cur], &dom->view[nxt], &dom->cfg, &dom->layout, dom->steps + 1,
-                                     CGLBM_KERNEL_MARCH, &m, dom->stream))
+                                     CGLBM_KERNEL_MARCH, &m, dom->stream, dom->storage, dom->fstride))
                 return rc;
             HTRY(hipEventRecord(dom->ev0, dom->stream));
             for (int k = 0; k < steps_per_candidate; ++k)
                 if (int rc = launch_step(&dom->view[dom->cur], &dom->view[nxt], &dom->cfg, &dom->layout,
-                                         dom->steps + 1, CGLBM_KERNEL_MARCH, &m, dom->stream))
+                                         dom->steps + 1, CGLBM_KERNEL_MARCH, &m, dom->stream, dom->storage,
+                                         dom->fstride))
                     return rc;
             HTRY(hipEventRecord(dom->ev1, dom->stream));
             HTRY(hipEventSynchronize(dom->ev1));
@@ -258,8 +298,7 @@ int cglbm_init_bubble(cglbm_domain *dom, double x0, double y0, double z0, double
             }
     dom->steps = 0;
     if (dom->batch) batch_member_changed(dom->batch);
-    if (int rc = upload(dom, h)) return rc;
-    return apply_ybc_inplace(dom);
+    return upload_init(dom, h, true);
 }
 
 // launch_InitJet (launch.cuh:52-72) followed by main.cu:26's boundary call with B == Aold.
@@ -272,8 +311,7 @@ int cglbm_init_jet(cglbm_domain *dom) {
     for (int64_t i = 0; i < n; ++i) h[i] = 1.0f;  // red at rest (jetIn.cuh:16-32)
     dom->steps = 0;
     if (dom->batch) batch_member_changed(dom->batch);
-    if (int rc = upload(dom, h)) return rc;
-    return apply_ybc_inplace(dom);
+    return upload_init(dom, h, true);
 }
 
 // launch_InitRTI (launch.cuh:42-46): rtiIn.cuh:18-71 evaluated in double (plus optional seeded
@@ -317,9 +355,7 @@ int cglbm_init_rti(cglbm_domain *dom, double amplitude, double width, int32_t n_
         }
     dom->steps = 0;
     if (dom->batch) batch_member_changed(dom->batch);
-    if (int rc = upload(dom, h)) return rc;
-    if (c.ybc == CGLBM_Y_WALLS) return apply_ybc_inplace(dom);
-    return CGLBM_OK;
+    return upload_init(dom, h, c.ybc == CGLBM_Y_WALLS);
 }
 
 // main.cu:38-55: update, boundary planes, swap.
@@ -330,7 +366,7 @@ int cglbm_step(cglbm_domain *dom, int32_t nsteps) {
     for (int k = 0; k < nsteps; ++k) {
         const int nxt = 1 - dom->cur;
         if (int rc = launch_step(&dom->view[dom->cur], &dom->view[nxt], &dom->cfg, &dom->layout, dom->steps + 1,
-                                 dom->kernel, &dom->shape, dom->stream))
+                                 dom->kernel, &dom->shape, dom->stream, dom->storage, dom->fstride))
             return rc;
         dom->cur = nxt;
         ++dom->steps;
@@ -364,7 +400,14 @@ int cglbm_copy_out(cglbm_domain *dom, uint32_t field_mask, float *host) {
     if (int rc = wait_batch(dom)) return rc;
     if (!host || (field_mask & ~0x7FFu)) return set_error(CGLBM_EINVAL, "bad arguments");
     const int64_t n = stored_cells(dom->cfg);
-    float *const *v = reinterpret_cast<float *const *>(&dom->view[dom->cur]);
+    Staging st;
+    const cglbm_moments *m = &dom->view[dom->cur];
+    if (dom->storage == CGLBM_STORE_F16 && (field_mask & ~0x3u)) {
+        if (int rc = make_staging(dom, st)) return rc;
+        HTRY(launch_unpack_f16(dom->buf[dom->cur], st.p, dom->fstride, n, dom->stream));
+        m = &st.view;
+    }
+    float *const *v = reinterpret_cast<float *const *>(m);
     int k = 0;
     for (int f = 0; f < 11; ++f)
         if (field_mask & (1u << f))
@@ -378,11 +421,20 @@ int cglbm_copy_in(cglbm_domain *dom, uint32_t field_mask, const float *host) {
     if (int rc = wait_batch(dom)) return rc;
     if (!host || (field_mask & ~0x7FFu)) return set_error(CGLBM_EINVAL, "bad arguments");
     const int64_t n = stored_cells(dom->cfg);
-    float *const *v = reinterpret_cast<float *const *>(&dom->view[dom->cur]);
+    Staging st;
+    const cglbm_moments *m = &dom->view[dom->cur];
+    const bool f16 = dom->storage == CGLBM_STORE_F16 && (field_mask & ~0x3u);
+    if (f16) {  // decode, overwrite the selected fields, encode (unselected fields keep their values)
+        if (int rc = make_staging(dom, st)) return rc;
+        HTRY(launch_unpack_f16(dom->buf[dom->cur], st.p, dom->fstride, n, dom->stream));
+        m = &st.view;
+    }
+    float *const *v = reinterpret_cast<float *const *>(m);
     int k = 0;
     for (int f = 0; f < 11; ++f)
         if (field_mask & (1u << f))
             HTRY(hipMemcpyAsync(v[f], host + (k++) * n, sizeof(float) * n, hipMemcpyHostToDevice, dom->stream));
+    if (f16) HTRY(launch_pack_f16(st.p, dom->buf[dom->cur], dom->fstride, n, dom->stream));
     HTRY(hipStreamSynchronize(dom->stream));
     return CGLBM_OK;
 }
@@ -392,9 +444,15 @@ int cglbm_reduce(cglbm_domain *dom, double out[6]) {
     if (int rc = wait_batch(dom)) return rc;
     if (!out) return set_error(CGLBM_EINVAL, "null output");
     const cglbm_config &c = dom->cfg;
-    float *const *v = reinterpret_cast<float *const *>(&dom->view[dom->cur]);
-    HTRY(launch_reduce(v, c.nx, c.ny, c.nz, c.ybc == CGLBM_Y_PERIODIC ? 0 : 1, static_cast<uint32_t>(dom->layout.pitch_y),
-                       static_cast<uint32_t>(dom->layout.pitch_z), dom->d_partials, dom->d_result, dom->stream));
+    const int y0 = c.ybc == CGLBM_Y_PERIODIC ? 0 : 1;
+    const uint32_t py = static_cast<uint32_t>(dom->layout.pitch_y), pz = static_cast<uint32_t>(dom->layout.pitch_z);
+    if (dom->storage == CGLBM_STORE_F16) {
+        HTRY(launch_reduce_f16(dom->buf[dom->cur], dom->fstride, c.nx, c.ny, c.nz, y0, py, pz, dom->d_partials,
+                               dom->d_result, dom->stream));
+    } else {
+        float *const *v = reinterpret_cast<float *const *>(&dom->view[dom->cur]);
+        HTRY(launch_reduce(v, c.nx, c.ny, c.nz, y0, py, pz, dom->d_partials, dom->d_result, dom->stream));
+    }
     HTRY(hipMemcpyAsync(out, dom->d_result, sizeof(double) * 6, hipMemcpyDeviceToHost, dom->stream));
     HTRY(hipStreamSynchronize(dom->stream));
     return CGLBM_OK;

@@ -32,6 +32,7 @@ struct MarchArgs {
     float jet_x0, jet_z0, jet_r, jet_u, jet_noise;
     uint32_t seed;
     int64_t step;
+    int store;  // kStoreF32 or kStoreF16 (storage.hpp); spans and strides refer to that layout
 };
 // Batched launch: descriptors of member domains (one kernel class: q, ybc, forced), a list of
 // work items {domain, tile x, tile y, chunk}, and the device step base.
@@ -50,28 +51,43 @@ struct BatchLaunch {
     const int64_t *base;
     int k;
     int grid;
-    int ybc, forced;
+    int ybc, forced, store;
     MarchShape shape;
 };
-hipError_t launch_batch_q19(const BatchLaunch &L, hipStream_t s);
-hipError_t launch_batch_q27(const BatchLaunch &L, hipStream_t s);
+hipError_t launch_batch_q19_f32(const BatchLaunch &L, hipStream_t s);
+hipError_t launch_batch_q19_f16(const BatchLaunch &L, hipStream_t s);
+hipError_t launch_batch_q27_f32(const BatchLaunch &L, hipStream_t s);
+hipError_t launch_batch_q27_f16(const BatchLaunch &L, hipStream_t s);
+hipError_t launch_batch(const BatchLaunch &L, int q, hipStream_t s);
 // device base += n (one thread), ordered on the stream after a bundle of n steps
 hipError_t launch_advance(int64_t *base, int n, hipStream_t s);
 
-// Fills `m` when the 11 fields of each state sit at one stride (the domain layout, or caller
-// arrays carved from one allocation); false otherwise.
+// Fills `m` (FP32 storage) when the 11 fields of each state sit at one stride (the domain layout,
+// or caller arrays carved from one allocation); false otherwise.
 bool march_args(const StepArgs &a, MarchArgs &m);
+// Same geometry and physics for an FP16-deviation state pair at field stride fs (storage.hpp).
+bool march_args_f16(const StepArgs &a, const float *A0, float *B0, int64_t fs, MarchArgs &m);
 bool march_supports(const StepArgs &a, int q);
 MarchShape march_default_shape(const StepArgs &a, int q);
 hipError_t launch_march_shape(const StepArgs &a, int q, MarchShape m, hipStream_t s);
-hipError_t launch_march_q19(const MarchArgs &a, MarchShape m, hipStream_t s);
-hipError_t launch_march_q27(const MarchArgs &a, MarchShape m, hipStream_t s);
+hipError_t launch_march_args(const MarchArgs &a, int q, MarchShape m, hipStream_t s);
+hipError_t launch_march_q19_f32(const MarchArgs &a, MarchShape m, hipStream_t s);
+hipError_t launch_march_q19_f16(const MarchArgs &a, MarchShape m, hipStream_t s);
+hipError_t launch_march_q27_f32(const MarchArgs &a, MarchShape m, hipStream_t s);
+hipError_t launch_march_q27_f16(const MarchArgs &a, MarchShape m, hipStream_t s);
 hipError_t launch_march(const StepArgs &a, int q, hipStream_t s);
+
+// FP16-deviation states (storage.hpp) <-> FP32 compact fields (11 x fs floats, field f at f fs).
+hipError_t launch_pack_f16(const float *src, float *state, int64_t fs, int64_t n, hipStream_t s);
+hipError_t launch_unpack_f16(const float *state, float *dst, int64_t fs, int64_t n, hipStream_t s);
 
 // FP64 diagnostics: partial sums per block into `partials` (capacity kReduceBlocks * 6), then the
 // fixed-order final sum into `result` (6 doubles).
 inline constexpr int kReduceBlocks = 512;
 hipError_t launch_reduce(const float *const *A, int nx, int ny, int nz, int y0, uint32_t py, uint32_t pz,
                          double *partials, double *result, hipStream_t s);
+// Same over an FP16-deviation state (storage.hpp) at field stride fs.
+hipError_t launch_reduce_f16(const float *state, int64_t fs, int nx, int ny, int nz, int y0, uint32_t py, uint32_t pz,
+                             double *partials, double *result, hipStream_t s);
 
 }  // namespace cglbm

@@ -26,6 +26,7 @@
 
 #include "kernels.hpp"
 #include "march_core.hpp"
+#include "storage.hpp"
 
 #ifndef CGLBM_MARCH_PREFETCH
 #define CGLBM_MARCH_PREFETCH 1
@@ -77,9 +78,12 @@ struct Held {
     float2 v[Q];
 };
 
-// One state as a buffer resource: field f of cell `id` at byte f * fs4 + 4 * id. Loads and
-// stores take the cell offset in a VGPR and the field offset in an SGPR, so the 11 accesses of a
-// cell need no per-field address arithmetic.
+// One state as a buffer resource. FP32 storage: field f of cell `id` at byte 4 (f fs + id). FP16
+// storage (storage.hpp): rhor/rhob as FP32, field f >= 2 as a scaled binary16 at byte
+// 2 (fs (f + 2) + id). Loads and stores take the cell offset in a VGPR and the field offset in an
+// SGPR, so the 11 accesses of a cell need no per-field address arithmetic. The field index is a
+// compile-time constant at every call site after unrolling, so the format branch folds away.
+template <int ST>
 struct FieldBuf {
     __amdgpu_buffer_rsrc_t r;
     uint32_t fs4;
@@ -89,9 +93,18 @@ struct FieldBuf {
         fs4 = fs * 4u;
     }
     __device__ __forceinline__ float ld(int f, uint32_t id) const {
+        if (ST == kStoreF16 && f >= 2) {
+            const uint32_t soff = static_cast<uint32_t>(f + 2) * (fs4 >> 1);
+            return f16_decode(f, static_cast<uint16_t>(__builtin_amdgcn_raw_buffer_load_b16(r, id * 2u, soff, 0)));
+        }
         return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, id * 4u, static_cast<uint32_t>(f) * fs4, 0));
     }
     __device__ __forceinline__ void st(int f, uint32_t id, float v) const {
+        if (ST == kStoreF16 && f >= 2) {
+            const uint32_t soff = static_cast<uint32_t>(f + 2) * (fs4 >> 1);
+            __builtin_amdgcn_raw_buffer_store_b16(f16_encode(f, v), r, id * 2u, soff, 0);
+            return;
+        }
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, id * 4u, static_cast<uint32_t>(f) * fs4, 0);
     }
     __device__ __forceinline__ Mom mom(uint32_t id) const {
@@ -114,7 +127,8 @@ struct FieldBuf {
 // Boundary planes of the new state (jet/boundary_conditions.cuh:16-109 or the wall copies of
 // rti/rtiIn.cuh:77-119), written by the lane that has just finished row y = 1 (inlet/bottom) or
 // y = ny (outlet/top) of column (gx, zt).
-__device__ __forceinline__ void boundary_planes(const MarchArgs &a, const FieldBuf &A, const FieldBuf &B, int gx,
+template <int ST>
+__device__ __forceinline__ void boundary_planes(const MarchArgs &a, const FieldBuf<ST> &A, const FieldBuf<ST> &B, int gx,
                                                 int zt, int rown, const float out[11]) {
     auto cell = [&](int row) {
         return static_cast<uint32_t>(gx) + a.py * static_cast<uint32_t>(row) + a.pz * static_cast<uint32_t>(zt);
@@ -188,7 +202,7 @@ enum : int { kRoleOwn = 0, kRoleRows = 1, kRoleCols = 2, kRoleNone = 3 };
 // One work item: tile (tix, tiy) of chunk `chunk` (of nchunk) of the domain described by `a`,
 // executed by the whole work-group. Ends with the work-group synchronised so the caller may run
 // another item with the same LDS.
-template <int Q, int YBC, int TY, int NBUF, bool FORCED>
+template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST>
 __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBUF> &sh, int tix, int tiy, int chunk,
                                            int nchunk) {
     constexpr int NOWN = TX * TY;
@@ -204,8 +218,8 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     const int x0 = tix * TX, r0 = tiy * TY;
     const int t = threadIdx.x;
     const Coeffs c = a.c;
-    const FieldBuf Abuf(a.A0, a.fsA, a.spanA);
-    const FieldBuf Bbuf(a.B0, a.fsB, a.spanB);
+    const FieldBuf<ST> Abuf(a.A0, a.fsA, a.spanA);
+    const FieldBuf<ST> Bbuf(a.B0, a.fsB, a.spanB);
 
     // ---- role and source cell of this lane -----------------------------------------------
     int role, sx, sy;
@@ -437,7 +451,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     if constexpr (NBUF == 2) __syncthreads();
 }
 
-template <int Q, int YBC, int TY, int NBUF, bool FORCED>
+template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST>
 __global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
     march_kernel(MarchArgs a, int ntx, int nty, int nchunk, int nblocks) {
     __shared__ Shared<Q, TY, NBUF> sh;
@@ -448,7 +462,7 @@ __global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
     const int tix = logical % ntx;
     const int tiy = (logical / ntx) % nty;
     const int chunk = logical / (ntx * nty);
-    march_tile<Q, YBC, TY, NBUF, FORCED>(a, sh, tix, tiy, chunk, nchunk);
+    march_tile<Q, YBC, TY, NBUF, FORCED, ST>(a, sh, tix, tiy, chunk, nchunk);
 }
 
 // Batched form: many domains of one kernel class in one launch. Each work item names a domain
@@ -456,7 +470,7 @@ __global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
 // items remain, so any grid size covers the list. The step counter (parity of the ping-pong and
 // the inlet perturbation key) is base[0] + k, base living in device memory so a captured
 // multi-step graph can be replayed.
-template <int Q, int YBC, int TY, int NBUF, bool FORCED>
+template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST>
 __global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
     march_batch_kernel(const BatchDesc *__restrict__ descs, const int4 *__restrict__ items, int nitems,
                        const int64_t *__restrict__ base, int k) {
@@ -474,11 +488,11 @@ __global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
         a.A0 = parity ? d.buf1 : d.buf0;
         a.B0 = parity ? d.buf0 : d.buf1;
         a.step = steps_done + 1;
-        march_tile<Q, YBC, TY, NBUF, FORCED>(a, sh, it.y, it.z, it.w, d.nchunk);
+        march_tile<Q, YBC, TY, NBUF, FORCED, ST>(a, sh, it.y, it.z, it.w, d.nchunk);
     }
 }
 
-template <int Q, int YBC, int TY, int NBUF>
+template <int Q, int YBC, int TY, int NBUF, int ST>
 hipError_t launch_shape(const MarchArgs &a, int L, hipStream_t s) {
     const int ntx = (a.nx + TX - 1) / TX;
     const int nty = (a.ny + TY - 1) / TY;
@@ -489,11 +503,11 @@ hipError_t launch_shape(const MarchArgs &a, int L, hipStream_t s) {
     // the gather path (march_supports).
     if constexpr (YBC != kYPeriodic) {
         if (a.forced) {
-            march_kernel<Q, YBC, TY, NBUF, true><<<grid, TX * TY + 128, 0, s>>>(a, ntx, nty, nchunk, nblocks);
+            march_kernel<Q, YBC, TY, NBUF, true, ST><<<grid, TX * TY + 128, 0, s>>>(a, ntx, nty, nchunk, nblocks);
             return hipGetLastError();
         }
     }
-    march_kernel<Q, YBC, TY, NBUF, false><<<grid, TX * TY + 128, 0, s>>>(a, ntx, nty, nchunk, nblocks);
+    march_kernel<Q, YBC, TY, NBUF, false, ST><<<grid, TX * TY + 128, 0, s>>>(a, ntx, nty, nchunk, nblocks);
     return hipGetLastError();
 }
 
@@ -503,53 +517,56 @@ constexpr bool fits2() {
     return shared_bytes<Q, TY, 2>() <= 160 * 1024;
 }
 
-template <int Q, int YBC>
+template <int Q, int YBC, int ST>
 hipError_t launch_q(const MarchArgs &a, MarchShape m, hipStream_t s) {
     const bool one = m.nbuf == 1;
     switch (m.ty) {
-    case 4: return one ? launch_shape<Q, YBC, 4, 1>(a, m.L, s) : launch_shape<Q, YBC, 4, 2>(a, m.L, s);
+    case 4: return one ? launch_shape<Q, YBC, 4, 1, ST>(a, m.L, s) : launch_shape<Q, YBC, 4, 2, ST>(a, m.L, s);
     case 16:
         if constexpr (fits2<Q, 16>()) {
-            if (!one) return launch_shape<Q, YBC, 16, 2>(a, m.L, s);
+            if (!one) return launch_shape<Q, YBC, 16, 2, ST>(a, m.L, s);
         }
-        return launch_shape<Q, YBC, 16, 1>(a, m.L, s);
-    default: return one ? launch_shape<Q, YBC, 8, 1>(a, m.L, s) : launch_shape<Q, YBC, 8, 2>(a, m.L, s);
+        return launch_shape<Q, YBC, 16, 1, ST>(a, m.L, s);
+    default: return one ? launch_shape<Q, YBC, 8, 1, ST>(a, m.L, s) : launch_shape<Q, YBC, 8, 2, ST>(a, m.L, s);
     }
 }
 
-template <int Q>
-hipError_t launch_march_q(const MarchArgs &a, MarchShape m, hipStream_t s) {
-    return a.ybc == kYPeriodic ? launch_q<Q, kYPeriodic>(a, m, s) : launch_q<Q, kYJet>(a, m, s);
+// One stencil and storage format (instantiated per translation unit: march_q19.hip, ...).
+template <int Q, int ST>
+hipError_t launch_march_qs(const MarchArgs &a, MarchShape m, hipStream_t s) {
+    return a.ybc == kYPeriodic ? launch_q<Q, kYPeriodic, ST>(a, m, s) : launch_q<Q, kYJet, ST>(a, m, s);
 }
 
-template <int Q, int YBC, int TY, int NBUF, bool FORCED>
+template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST>
 hipError_t launch_batch_shape(const BatchLaunch &L, hipStream_t s) {
-    march_batch_kernel<Q, YBC, TY, NBUF, FORCED>
+    march_batch_kernel<Q, YBC, TY, NBUF, FORCED, ST>
         <<<L.grid, TX * TY + 128, 0, s>>>(L.descs, L.items, L.nitems, L.base, L.k);
     return hipGetLastError();
 }
 
-template <int Q, int YBC, bool FORCED>
+template <int Q, int YBC, bool FORCED, int ST>
 hipError_t launch_batch_qy(const BatchLaunch &L, hipStream_t s) {
     const bool one = L.shape.nbuf == 1;
     switch (L.shape.ty) {
     case 4:
-        return one ? launch_batch_shape<Q, YBC, 4, 1, FORCED>(L, s) : launch_batch_shape<Q, YBC, 4, 2, FORCED>(L, s);
+        return one ? launch_batch_shape<Q, YBC, 4, 1, FORCED, ST>(L, s)
+                   : launch_batch_shape<Q, YBC, 4, 2, FORCED, ST>(L, s);
     case 16:
         if constexpr (fits2<Q, 16>()) {
-            if (!one) return launch_batch_shape<Q, YBC, 16, 2, FORCED>(L, s);
+            if (!one) return launch_batch_shape<Q, YBC, 16, 2, FORCED, ST>(L, s);
         }
-        return launch_batch_shape<Q, YBC, 16, 1, FORCED>(L, s);
+        return launch_batch_shape<Q, YBC, 16, 1, FORCED, ST>(L, s);
     default:
-        return one ? launch_batch_shape<Q, YBC, 8, 1, FORCED>(L, s) : launch_batch_shape<Q, YBC, 8, 2, FORCED>(L, s);
+        return one ? launch_batch_shape<Q, YBC, 8, 1, FORCED, ST>(L, s)
+                   : launch_batch_shape<Q, YBC, 8, 2, FORCED, ST>(L, s);
     }
 }
 
-template <int Q>
-hipError_t launch_batch_q(const BatchLaunch &L, hipStream_t s) {
-    if (L.ybc == kYPeriodic) return launch_batch_qy<Q, kYPeriodic, false>(L, s);
-    if (L.forced) return launch_batch_qy<Q, kYJet, true>(L, s);
-    return launch_batch_qy<Q, kYJet, false>(L, s);
+template <int Q, int ST>
+hipError_t launch_batch_qs(const BatchLaunch &L, hipStream_t s) {
+    if (L.ybc == kYPeriodic) return launch_batch_qy<Q, kYPeriodic, false, ST>(L, s);
+    if (L.forced) return launch_batch_qy<Q, kYJet, true, ST>(L, s);
+    return launch_batch_qy<Q, kYJet, false, ST>(L, s);
 }
 
 }  // namespace march

@@ -4,6 +4,7 @@
 // block partials in index order. The result therefore does not depend on timing or on what else
 // runs on the GPU.
 #include "kernels.hpp"
+#include "storage.hpp"
 
 namespace cglbm {
 
@@ -11,8 +12,16 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__global__ void __launch_bounds__(kThreads) reduce_partials_kernel(const float *rr, const float *rb, const float *ux,
-                                                                   const float *uy, const float *uz, int nx, int ny,
+// Velocity component of field f (2..4) at cell id: FP32, or a scaled binary16 (storage.hpp).
+template <int ST>
+__device__ __forceinline__ float load_u(const void *p, int f, uint32_t id) {
+    if constexpr (ST == kStoreF16) return f16_decode(f, static_cast<const uint16_t *>(p)[id]);
+    else return static_cast<const float *>(p)[id];
+}
+
+template <int ST>
+__global__ void __launch_bounds__(kThreads) reduce_partials_kernel(const float *rr, const float *rb, const void *pux,
+                                                                   const void *puy, const void *puz, int nx, int ny,
                                                                    int nz, int y0, uint32_t py, uint32_t pz,
                                                                    double *partials) {
     __shared__ double sh[6][kThreads];
@@ -30,13 +39,13 @@ __global__ void __launch_bounds__(kThreads) reduce_partials_kernel(const float *
         const uint32_t id = static_cast<uint32_t>(x) + py * static_cast<uint32_t>(y) + pz * static_cast<uint32_t>(z);
         const double a = rr[id], b = rb[id];
         const double rho = a + b;
+        const double ux = load_u<ST>(pux, 2, id), uy = load_u<ST>(puy, 3, id), uz = load_u<ST>(puz, 4, id);
         s[0] += a;
         s[1] += b;
-        s[2] += rho * static_cast<double>(ux[id]);
-        s[3] += rho * static_cast<double>(uy[id]);
-        s[4] += rho * static_cast<double>(uz[id]);
-        const bool bad = !(a >= 0.0) || !(b >= 0.0) || !(rho > 0.0) || !isfinite(static_cast<double>(ux[id])) ||
-                         !isfinite(static_cast<double>(uy[id])) || !isfinite(static_cast<double>(uz[id]));
+        s[2] += rho * ux;
+        s[3] += rho * uy;
+        s[4] += rho * uz;
+        const bool bad = !(a >= 0.0) || !(b >= 0.0) || !(rho > 0.0) || !isfinite(ux) || !isfinite(uy) || !isfinite(uz);
         s[5] += bad ? 1.0 : 0.0;
     }
 #pragma unroll
@@ -77,8 +86,18 @@ __global__ void __launch_bounds__(kThreads) reduce_final_kernel(const double *pa
 
 hipError_t launch_reduce(const float *const *A, int nx, int ny, int nz, int y0, uint32_t py, uint32_t pz,
                          double *partials, double *result, hipStream_t s) {
-    reduce_partials_kernel<<<kReduceBlocks, kThreads, 0, s>>>(A[0], A[1], A[2], A[3], A[4], nx, ny, nz, y0, py, pz,
-                                                              partials);
+    reduce_partials_kernel<kStoreF32><<<kReduceBlocks, kThreads, 0, s>>>(A[0], A[1], A[2], A[3], A[4], nx, ny, nz, y0,
+                                                                         py, pz, partials);
+    reduce_final_kernel<<<1, kThreads, 0, s>>>(partials, kReduceBlocks, result);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_f16(const float *state, int64_t fs, int nx, int ny, int nz, int y0, uint32_t py, uint32_t pz,
+                             double *partials, double *result, hipStream_t s) {
+    const char *b = reinterpret_cast<const char *>(state);
+    reduce_partials_kernel<kStoreF16><<<kReduceBlocks, kThreads, 0, s>>>(
+        state, state + fs, b + field_offset(kStoreF16, 2, fs), b + field_offset(kStoreF16, 3, fs),
+        b + field_offset(kStoreF16, 4, fs), nx, ny, nz, y0, py, pz, partials);
     reduce_final_kernel<<<1, kThreads, 0, s>>>(partials, kReduceBlocks, result);
     return hipGetLastError();
 }

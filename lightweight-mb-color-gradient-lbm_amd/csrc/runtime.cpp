@@ -23,6 +23,7 @@
 #include "kernels.hpp"
 #include "lattice.hpp"
 #include "state.hpp"
+#include "storage.hpp"
 
 using namespace cglbm;
 
@@ -171,8 +172,22 @@ void tuned_shape_store(const cglbm_config &c, MarchShape m) {
 }
 
 int launch_step(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
-                const cglbm_layout *L, int64_t step, int kernel, const MarchShape *shape, hipStream_t s) {
+                const cglbm_layout *L, int64_t step, int kernel, const MarchShape *shape, hipStream_t s, int store,
+                int64_t fs) {
     StepArgs a = make_step_args(A, B, cfg, L, step);
+    if (store == kStoreF16) {
+        // Only the z-marching kernel reads and writes the FP16-deviation format.
+        MarchArgs ma;
+        if (kernel == CGLBM_KERNEL_GATHER || (a.forced && a.ybc == CGLBM_Y_PERIODIC) || a.nx < 3 || a.ny < 3 ||
+            a.nz < 3 || !march_args_f16(a, A->rhor, B->rhor, fs, ma))
+            return fail(CGLBM_EINVAL, "FP16 storage needs the z-marching kernel and a supported case");
+        MarchShape m;
+        if (shape && shape->ty > 0) m = *shape;
+        else if (!tuned_shape_lookup(*cfg, &m)) m = march_default_shape(a, cfg->q);
+        const hipError_t e = launch_march_args(ma, cfg->q, m, s);
+        if (e != hipSuccess) return hip_fail(e, "launch_march (FP16 storage)");
+        return CGLBM_OK;
+    }
     const bool march_ok = march_supports(a, cfg->q);
     if (kernel == CGLBM_KERNEL_MARCH && !march_ok)
         return fail(CGLBM_EINVAL, "the z-marching kernel does not support this shape/layout");

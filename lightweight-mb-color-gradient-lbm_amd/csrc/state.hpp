@@ -17,8 +17,11 @@ StepArgs make_step_args(const cglbm_moments *A, const cglbm_moments *B, const cg
 BcArgs make_bc_args(const cglbm_moments *B, const cglbm_moments *Aold, const cglbm_config *c,
                     const cglbm_layout *L, int64_t step);
 // shape == nullptr: the tuned shape recorded for this configuration, else the default.
+// store == kStoreF16: A->rhor / B->rhor are the bases of FP16-deviation states at field stride fs
+// (storage.hpp); the other view pointers are not used.
 int launch_step(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
-                const cglbm_layout *L, int64_t step, int kernel, const MarchShape *shape, hipStream_t s);
+                const cglbm_layout *L, int64_t step, int kernel, const MarchShape *shape, hipStream_t s,
+                int store = 0, int64_t fs = 0);
 
 // Process-wide record of autotuned z-marching shapes, keyed by (q, ybc, nx, ny, nz).
 bool tuned_shape_lookup(const cglbm_config &c, MarchShape *out);

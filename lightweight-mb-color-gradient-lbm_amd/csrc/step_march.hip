@@ -3,6 +3,7 @@
 #include <cstdlib>
 
 #include "kernels.hpp"
+#include "storage.hpp"
 
 namespace cglbm {
 
@@ -57,6 +58,20 @@ bool march_args(const StepArgs &a, MarchArgs &m) {
     m.jet_noise = a.jet_noise;
     m.seed = a.seed;
     m.step = a.step;
+    m.store = kStoreF32;
+    return true;
+}
+
+bool march_args_f16(const StepArgs &a, const float *A0, float *B0, int64_t fs, MarchArgs &m) {
+    if (fs <= 0 || fs % 64 != 0 || state_bytes(kStoreF16, fs) >= (int64_t(1) << 32)) return false;
+    StepArgs u = a;  // uniform-stride stand-in so march_args fills the geometry and physics
+    for (int f = 0; f < 11; ++f) {
+        u.A[f] = A0 + f * fs;
+        u.B[f] = B0 + f * fs;
+    }
+    if (!march_args(u, m)) return false;
+    m.store = kStoreF16;
+    m.spanA = m.spanB = static_cast<uint32_t>(state_bytes(kStoreF16, fs));
     return true;
 }
 
@@ -70,7 +85,20 @@ hipError_t launch_march_shape(const StepArgs &a, int q, MarchShape m, hipStream_
     if ((m.ty != 4 && m.ty != 8 && m.ty != 16) || m.L < 1 || (m.nbuf != 1 && m.nbuf != 2)) return hipErrorInvalidValue;
     MarchArgs ma;
     if (!march_args(a, ma)) return hipErrorInvalidValue;
-    return q == 19 ? launch_march_q19(ma, m, s) : launch_march_q27(ma, m, s);
+    return launch_march_args(ma, q, m, s);
+}
+
+hipError_t launch_march_args(const MarchArgs &a, int q, MarchShape m, hipStream_t s) {
+    if ((m.ty != 4 && m.ty != 8 && m.ty != 16) || m.L < 1 || (m.nbuf != 1 && m.nbuf != 2)) return hipErrorInvalidValue;
+    if (q == 19) return a.store == kStoreF16 ? launch_march_q19_f16(a, m, s) : launch_march_q19_f32(a, m, s);
+    if (q == 27) return a.store == kStoreF16 ? launch_march_q27_f16(a, m, s) : launch_march_q27_f32(a, m, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch(const BatchLaunch &L, int q, hipStream_t s) {
+    if (q == 19) return L.store == kStoreF16 ? launch_batch_q19_f16(L, s) : launch_batch_q19_f32(L, s);
+    if (q == 27) return L.store == kStoreF16 ? launch_batch_q27_f16(L, s) : launch_batch_q27_f32(L, s);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_march(const StepArgs &a, int q, hipStream_t s) {

@@ -136,6 +136,33 @@ def run(p: Params, A: np.ndarray, nsteps: int, first_step: int = 1) -> np.ndarra
     return a
 
 
+# FP16-deviation storage: rhor/rhob kept as float32; fields 2..4 (u) stored as binary16 of u * 2^10,
+# fields 5..10 (Pi) as binary16 of Pi * 2^12, rounded to nearest even (numpy's float32 -> float16
+# cast). Decoding multiplies back by the exact inverse power of two.
+F16_SCALE = np.array([1, 1] + [1024] * 3 + [4096] * 6, dtype=np.float32)
+
+
+def quantize_f16(A: np.ndarray) -> np.ndarray:
+    """The float32 state a FP16-deviation store would hold for A (decoded back to float32)."""
+    A = np.asarray(A, dtype=np.float32)
+    out = A.copy()
+    with np.errstate(over="ignore"):  # beyond the stored range a value becomes +-inf, as on the device
+        for f in range(2, 11):
+            s = F16_SCALE[f]
+            out[f] = (A[f] * s).astype(np.float16).astype(np.float32) / s
+    return out
+
+
+def run_f16(p: Params, A: np.ndarray, nsteps: int, first_step: int = 1) -> np.ndarray:
+    """nsteps float32 updates, each followed by the FP16-deviation storage rounding."""
+    a = quantize_f16(A)
+    b = np.zeros_like(a)
+    for k in range(nsteps):
+        step(p, a, b, first_step + k)
+        a, b = quantize_f16(b), a
+    return a
+
+
 def apply_ybc(p: Params, B: np.ndarray, Aold: np.ndarray, step_index: int = 0):
     pc = p.c()
     fn = lib().cgref_apply_ybc_f32 if B.dtype == np.float32 else lib().cgref_apply_ybc_f64
