@@ -194,6 +194,9 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-domain batched measurement")
     ap.add_argument("--batched-steps", type=int, default=500)
+    ap.add_argument("--no-serve", action="store_true", help="skip the Poisson serving measurement")
+    ap.add_argument("--serve-rate", type=float, default=6.0, help="serving arrivals per second")
+    ap.add_argument("--serve-duration", type=float, default=60.0, help="serving arrival window (s)")
     ap.add_argument("--tile19", default=None, help="fixed D3Q19 tile 'rows,chunk,buffers' instead of autotuning")
     ap.add_argument("--tile27", default=None, help="fixed D3Q27 tile 'rows,chunk,buffers' instead of autotuning")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -221,6 +224,15 @@ def main() -> int:
         batched = measure_batched(cg, args.batched_steps, 10)
         print(f"bench: batched {batched['value']:.0f} MLUPS, {batched['speedup_vs_sequential']}x sequential",
               file=sys.stderr, flush=True)
+    serving = None
+    if not args.no_serve:
+        import cglbm.serving as sv
+        print(f"bench: serving mix, Poisson {args.serve_rate}/s over {args.serve_duration} s", file=sys.stderr,
+              flush=True)
+        serving = sv.run_poisson(cg, args.serve_rate, args.serve_duration, seed=0,
+                                 log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(f"bench: serving p50 {serving['latency_p50_s']:.2f} s p99 {serving['latency_p99_s']:.2f} s, "
+              f"{serving['aggregate_mlups']:.0f} MLUPS aggregate", file=sys.stderr, flush=True)
     if args.calibrate_copy:
         calibrate_copy(cg, args.calibrate_copy)
     workload = f"bubble{args.n}_d3q19"
@@ -259,6 +271,7 @@ def main() -> int:
         "mass_drift_rel": {"d3q19": r19["mass_drift_rel"], "d3q27": r27["mass_drift_rel"],
                            "d3q19_f16_storage": r16["mass_drift_rel"]},
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line))
     return 0

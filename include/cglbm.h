@@ -200,6 +200,83 @@ int cglbm_batch_time_sequential(cglbm_batch *b, int32_t nsteps, double *ms_total
 int cglbm_batch_autotune(cglbm_batch *b, int32_t steps_per_candidate);
 int cglbm_batch_sync(cglbm_batch *b);
 
+/* ---- Serving: continuous batching of independent cases ------------------------------------
+ * The reference has no request loop (one case per process, main.cu). A server admits requests
+ * (bubble, jet, RTI or a caller-given state; D3Q19/D3Q27; FP32 or FP16-deviation storage) in
+ * arrival order whenever the HBM budget and a member slot allow, creates and initialises each
+ * case on admission worker threads while the GPU steps the resident ones, keeps every state
+ * resident on the device, steps all of them together as one batch (graph-replayed bundles), and
+ * hands each case to a completion thread as soon as its step count is reached: results are
+ * copied to the caller's buffer, FP64 diagnostics computed, the device memory released. A
+ * case's result is identical to running it alone. Thread-safe: any thread may submit and poll. */
+
+typedef struct cglbm_server cglbm_server;
+
+enum { CGLBM_CASE_BUBBLE = 0, CGLBM_CASE_JET = 1, CGLBM_CASE_RTI = 2, CGLBM_CASE_STATE = 3 };
+
+typedef struct {
+    cglbm_config cfg;
+    int32_t storage;         /* CGLBM_STORE_F32 or CGLBM_STORE_F16 */
+    int32_t case_kind;       /* CGLBM_CASE_* */
+    double radius, width;    /* bubble: drop radius and tanh width (centre = domain middle); RTI: width */
+    int32_t sharp;           /* bubble: sharp sphere instead of the tanh profile */
+    double amplitude;        /* RTI: single-mode amplitude */
+    int32_t n_random_modes;  /* RTI: seeded random modes */
+    double random_amplitude;
+    uint32_t init_seed;
+    const float *initial;    /* CGLBM_CASE_STATE: 11 fields, compact layout (copied at submit) */
+    int64_t steps;           /* updates to run (>= 0) */
+    uint32_t result_mask;    /* fields copied to `result` when done (bit f = field f; 0 = none) */
+    float *result;           /* caller buffer for popcount(result_mask) fields, or NULL */
+    uint64_t user_tag;
+} cglbm_request;
+
+typedef struct {
+    uint64_t id;
+    uint64_t user_tag;
+    int32_t status;          /* CGLBM_OK or the CGLBM_E* code of the failure */
+    int32_t storage;
+    int64_t steps;           /* steps completed */
+    int64_t cells;           /* interior cells */
+    double diag[6];          /* cglbm_reduce of the final state */
+    double t_submit, t_admit, t_start, t_done;  /* seconds since cglbm_server_create */
+    char message[160];       /* error text when status != CGLBM_OK */
+} cglbm_result;
+
+typedef struct {
+    int64_t hbm_budget_bytes;  /* device bytes for resident cases (0: 80% of free memory) */
+    int32_t max_members;       /* resident-case slots (0: 1024) */
+    int32_t bundle;            /* steps per graph-replayed bundle (0: 8) */
+    int32_t admit_threads;     /* admission workers (0: 2) */
+    int32_t tile_ty, tile_L, tile_nbuf;  /* batch tile shape (tile_ty = 0: per-class default) */
+} cglbm_server_opts;
+
+typedef struct {
+    int64_t submitted, admitted, completed, failed;
+    int64_t queued, resident;  /* waiting for admission / on the device now */
+    int64_t bytes_reserved, peak_bytes_reserved, budget_bytes;
+    int64_t peak_resident;
+    int64_t updates;           /* sum of cells x steps of completed cases */
+    int64_t bundles;           /* batched step calls issued */
+    double busy_seconds;       /* time with at least one resident case being stepped */
+} cglbm_server_stats;
+
+int cglbm_server_create(const cglbm_server_opts *opts, cglbm_server **out);
+/* Validates and enqueues (never blocks on the GPU). CGLBM_ENOMEM if the case alone exceeds the
+ * budget. The request struct may be reused after the call; `result` must stay valid until the
+ * case's cglbm_result has been polled. */
+int cglbm_server_submit(cglbm_server *s, const cglbm_request *req, uint64_t *id);
+/* Up to `max` finished cases into `out` (*n set), waiting up to timeout_ms for the first one
+ * (0: no wait, < 0: wait indefinitely). */
+int cglbm_server_poll(cglbm_server *s, cglbm_result *out, int32_t max, int32_t timeout_ms, int32_t *n);
+/* Blocks until every submitted case has finished (results stay available to poll). */
+int cglbm_server_drain(cglbm_server *s);
+int cglbm_server_stats_get(cglbm_server *s, cglbm_server_stats *out);
+/* Seconds since cglbm_server_create (the clock of cglbm_result's timestamps). */
+double cglbm_server_clock(const cglbm_server *s);
+/* Stops the threads after the current work (pending cases are failed with CGLBM_ESTATE). */
+int cglbm_server_destroy(cglbm_server *s);
+
 #ifdef __cplusplus
 }
 #endif

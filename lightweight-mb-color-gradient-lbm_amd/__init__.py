@@ -41,7 +41,11 @@ EXPORTS = (
     "cglbm_batch_create", "cglbm_batch_destroy", "cglbm_batch_add", "cglbm_batch_remove", "cglbm_batch_size",
     "cglbm_batch_set_tile", "cglbm_batch_set_graph", "cglbm_batch_step", "cglbm_batch_time_steps",
     "cglbm_batch_time_sequential", "cglbm_batch_autotune", "cglbm_batch_sync",
+    "cglbm_server_create", "cglbm_server_submit", "cglbm_server_poll", "cglbm_server_drain",
+    "cglbm_server_stats_get", "cglbm_server_clock", "cglbm_server_destroy",
 )
+
+CASE_BUBBLE, CASE_JET, CASE_RTI, CASE_STATE = 0, 1, 2, 3
 
 
 class CglbmError(RuntimeError):
@@ -67,6 +71,41 @@ class _Config(ctypes.Structure):
         ("gravity", ctypes.c_float), ("atwood", ctypes.c_float),
         ("conservative_omega2", ctypes.c_int32),
     ]
+
+
+class _Request(ctypes.Structure):
+    _fields_ = [
+        ("cfg", _Config), ("storage", ctypes.c_int32), ("case_kind", ctypes.c_int32),
+        ("radius", ctypes.c_double), ("width", ctypes.c_double), ("sharp", ctypes.c_int32),
+        ("amplitude", ctypes.c_double), ("n_random_modes", ctypes.c_int32),
+        ("random_amplitude", ctypes.c_double), ("init_seed", ctypes.c_uint32),
+        ("initial", ctypes.c_void_p), ("steps", ctypes.c_int64), ("result_mask", ctypes.c_uint32),
+        ("result", ctypes.c_void_p), ("user_tag", ctypes.c_uint64),
+    ]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("id", ctypes.c_uint64), ("user_tag", ctypes.c_uint64), ("status", ctypes.c_int32),
+        ("storage", ctypes.c_int32), ("steps", ctypes.c_int64), ("cells", ctypes.c_int64),
+        ("diag", ctypes.c_double * 6), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+        ("t_start", ctypes.c_double), ("t_done", ctypes.c_double), ("message", ctypes.c_char * 160),
+    ]
+
+
+class _ServerOpts(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", ctypes.c_int64), ("max_members", ctypes.c_int32), ("bundle", ctypes.c_int32),
+        ("admit_threads", ctypes.c_int32), ("tile_ty", ctypes.c_int32), ("tile_L", ctypes.c_int32),
+        ("tile_nbuf", ctypes.c_int32),
+    ]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "submitted", "admitted", "completed", "failed", "queued", "resident", "bytes_reserved",
+        "peak_bytes_reserved", "budget_bytes", "peak_resident", "updates", "bundles")] + [
+        ("busy_seconds", ctypes.c_double)]
 
 
 @dataclass
@@ -170,6 +209,14 @@ def lib():
         "cglbm_batch_time_sequential": ([D, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "cglbm_batch_autotune": ([D, ctypes.c_int32], ctypes.c_int),
         "cglbm_batch_sync": ([D], ctypes.c_int),
+        "cglbm_server_create": ([ctypes.POINTER(_ServerOpts), ctypes.POINTER(D)], ctypes.c_int),
+        "cglbm_server_submit": ([D, ctypes.POINTER(_Request), ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+        "cglbm_server_poll": ([D, ctypes.POINTER(_Result), ctypes.c_int32, ctypes.c_int32,
+                               ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+        "cglbm_server_drain": ([D], ctypes.c_int),
+        "cglbm_server_stats_get": ([D, ctypes.POINTER(_ServerStats)], ctypes.c_int),
+        "cglbm_server_clock": ([D], ctypes.c_double),
+        "cglbm_server_destroy": ([D], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -390,3 +437,80 @@ class Batch:
 
     def sync(self):
         _check(lib().cglbm_batch_sync(self._h), "cglbm_batch_sync")
+
+
+class Server:
+    """Continuous-batching server (include/cglbm.h, cglbm_server_*): submit cases at any time;
+    they are admitted by HBM budget, stepped together while resident, and returned as results."""
+
+    _CASES = {"bubble": CASE_BUBBLE, "jet": CASE_JET, "rti": CASE_RTI, "state": CASE_STATE}
+
+    def __init__(self, hbm_budget_bytes: int = 0, max_members: int = 0, bundle: int = 0, admit_threads: int = 0,
+                 tile=None):
+        o = _ServerOpts(hbm_budget_bytes, max_members, bundle, admit_threads, *(tile or (0, 0, 0)))
+        h = ctypes.c_void_p()
+        _check(lib().cglbm_server_create(ctypes.byref(o), ctypes.byref(h)), "cglbm_server_create")
+        self._h = h
+        self._keep = {}  # id -> (result buffer, initial state) kept alive until polled
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().cglbm_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, cfg: Config, case: str = "bubble", steps: int = 100, storage: int = STORE_F32,
+               radius: float = 8.0, width: float = 4.0, sharp: bool = False, amplitude: float = 2.0,
+               n_random_modes: int = 0, random_amplitude: float = 0.0, init_seed: int = 0, initial=None,
+               result_mask: int = 0, tag: int = 0) -> int:
+        buf = None
+        if result_mask:
+            buf = np.empty((bin(result_mask).count("1"),) + cfg.shape[1:], dtype=np.float32)
+        init = None
+        if initial is not None:
+            init = np.ascontiguousarray(initial, dtype=np.float32)
+            if init.shape != cfg.shape:
+                raise CglbmError(f"initial state shape {init.shape} != {cfg.shape}")
+        r = _Request(cfg.c(), storage, self._CASES[case], radius, width, int(sharp), amplitude, n_random_modes,
+                     random_amplitude, init_seed, init.ctypes.data if init is not None else None, steps,
+                     result_mask, buf.ctypes.data if buf is not None else None, tag)
+        rid = ctypes.c_uint64()
+        _check(lib().cglbm_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)), "cglbm_server_submit")
+        self._keep[rid.value] = buf
+        return rid.value
+
+    def poll(self, max_results: int = 256, timeout_ms: int = 0):
+        """Finished cases as dicts (``fields`` holds the requested result fields)."""
+        arr = (_Result * max_results)()
+        n = ctypes.c_int32()
+        _check(lib().cglbm_server_poll(self._h, arr, max_results, timeout_ms, ctypes.byref(n)), "cglbm_server_poll")
+        out = []
+        for k in range(n.value):
+            r = arr[k]
+            out.append(dict(id=r.id, tag=r.user_tag, status=r.status, storage=r.storage, steps=r.steps,
+                            cells=r.cells, diag=np.array(r.diag[:]), t_submit=r.t_submit, t_admit=r.t_admit,
+                            t_start=r.t_start, t_done=r.t_done, message=r.message.decode(errors="replace"),
+                            fields=self._keep.pop(r.id, None)))
+        return out
+
+    def drain(self):
+        _check(lib().cglbm_server_drain(self._h), "cglbm_server_drain")
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check(lib().cglbm_server_stats_get(self._h, ctypes.byref(st)), "cglbm_server_stats_get")
+        return {name: getattr(st, name) for name, _ in _ServerStats._fields_}
+
+    def clock(self) -> float:
+        return float(lib().cglbm_server_clock(self._h))

@@ -216,6 +216,8 @@ void batch_member_changed(cglbm_batch *b) {
     if (b) b->dirty = true;
 }
 
+hipStream_t batch_stream(cglbm_batch *b) { return b ? b->stream : nullptr; }
+
 extern "C" {
 
 int cglbm_batch_create(cglbm_batch **out) {

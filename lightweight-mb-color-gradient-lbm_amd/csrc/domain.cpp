@@ -46,8 +46,9 @@ void make_views(cglbm_domain *d) {
     }
 }
 
-// Transient FP32 image of a state for an FP16 domain (11 x fstride floats, compact layout), with
-// its field views; freed by the destructor. Only the host copies and initialisers use it.
+// Transient FP32 area for an FP16 domain: nf fields of fstride floats (11 for the initialisers,
+// which assemble a whole FP32 state; 1 for the host copies, which convert one field at a time),
+// with field views when nf == 11; freed by the destructor.
 struct Staging {
     float *p = nullptr;
     cglbm_moments view{};
@@ -56,10 +57,12 @@ struct Staging {
     }
 };
 
-int make_staging(const cglbm_domain *d, Staging &st) {
-    HTRY(hipMalloc(&st.p, sizeof(float) * 11 * static_cast<size_t>(d->fstride)));
-    float **v = reinterpret_cast<float **>(&st.view);
-    for (int f = 0; f < 11; ++f) v[f] = st.p + f * d->fstride;
+int make_staging(const cglbm_domain *d, Staging &st, int nf) {
+    HTRY(hipMalloc(&st.p, sizeof(float) * nf * static_cast<size_t>(d->fstride)));
+    if (nf == 11) {
+        float **v = reinterpret_cast<float **>(&st.view);
+        for (int f = 0; f < 11; ++f) v[f] = st.p + f * d->fstride;
+    }
     return CGLBM_OK;
 }
 
@@ -89,7 +92,7 @@ int upload_init(cglbm_domain *d, const std::vector<float> &host, bool bc) {
     Staging st;
     const cglbm_moments *m = &d->view[d->cur];
     if (d->storage == CGLBM_STORE_F16) {
-        if (int rc = make_staging(d, st)) return rc;
+        if (int rc = make_staging(d, st, 11)) return rc;
         m = &st.view;
     }
     float *const *v = reinterpret_cast<float *const *>(m);
@@ -97,7 +100,8 @@ int upload_init(cglbm_domain *d, const std::vector<float> &host, bool bc) {
         HTRY(hipMemcpyAsync(v[f], host.data() + f * n, sizeof(float) * n, hipMemcpyHostToDevice, d->stream));
     if (bc && d->cfg.ybc != CGLBM_Y_PERIODIC)
         if (int rc = apply_ybc_views(d, *m)) return rc;
-    if (d->storage == CGLBM_STORE_F16) HTRY(launch_pack_f16(st.p, d->buf[d->cur], d->fstride, n, d->stream));
+    if (d->storage == CGLBM_STORE_F16)
+        HTRY(launch_pack_f16(st.p, d->fstride, d->buf[d->cur], d->fstride, n, 0, 11, d->stream));
     HTRY(hipStreamSynchronize(d->stream));
     return CGLBM_OK;
 }
@@ -394,24 +398,27 @@ int cglbm_sync(cglbm_domain *dom) {
     return CGLBM_OK;
 }
 
-// copy_out_D2H (memory.cuh:209-217), any subset of the 11 fields.
+// copy_out_D2H (memory.cuh:209-217), any subset of the 11 fields. FP16 states decode one field at
+// a time through a one-field staging area (stream order keeps each copy ahead of the next decode).
 int cglbm_copy_out(cglbm_domain *dom, uint32_t field_mask, float *host) {
     if (int rc = check_dom(dom)) return rc;
     if (int rc = wait_batch(dom)) return rc;
     if (!host || (field_mask & ~0x7FFu)) return set_error(CGLBM_EINVAL, "bad arguments");
     const int64_t n = stored_cells(dom->cfg);
+    const bool f16 = dom->storage == CGLBM_STORE_F16;
     Staging st;
-    const cglbm_moments *m = &dom->view[dom->cur];
-    if (dom->storage == CGLBM_STORE_F16 && (field_mask & ~0x3u)) {
-        if (int rc = make_staging(dom, st)) return rc;
-        HTRY(launch_unpack_f16(dom->buf[dom->cur], st.p, dom->fstride, n, dom->stream));
-        m = &st.view;
-    }
-    float *const *v = reinterpret_cast<float *const *>(m);
+    if (f16 && (field_mask & ~0x3u))
+        if (int rc = make_staging(dom, st, 1)) return rc;
     int k = 0;
-    for (int f = 0; f < 11; ++f)
-        if (field_mask & (1u << f))
-            HTRY(hipMemcpyAsync(host + (k++) * n, v[f], sizeof(float) * n, hipMemcpyDeviceToHost, dom->stream));
+    for (int f = 0; f < 11; ++f) {
+        if (!(field_mask & (1u << f))) continue;
+        const float *src = dom->buf[dom->cur] + f * dom->fstride;
+        if (f16 && f >= 2) {
+            HTRY(launch_unpack_f16(dom->buf[dom->cur], dom->fstride, st.p, dom->fstride, n, f, f + 1, dom->stream));
+            src = st.p;
+        }
+        HTRY(hipMemcpyAsync(host + (k++) * n, src, sizeof(float) * n, hipMemcpyDeviceToHost, dom->stream));
+    }
     HTRY(hipStreamSynchronize(dom->stream));
     return CGLBM_OK;
 }
@@ -421,20 +428,22 @@ int cglbm_copy_in(cglbm_domain *dom, uint32_t field_mask, const float *host) {
     if (int rc = wait_batch(dom)) return rc;
     if (!host || (field_mask & ~0x7FFu)) return set_error(CGLBM_EINVAL, "bad arguments");
     const int64_t n = stored_cells(dom->cfg);
+    const bool f16 = dom->storage == CGLBM_STORE_F16;
     Staging st;
-    const cglbm_moments *m = &dom->view[dom->cur];
-    const bool f16 = dom->storage == CGLBM_STORE_F16 && (field_mask & ~0x3u);
-    if (f16) {  // decode, overwrite the selected fields, encode (unselected fields keep their values)
-        if (int rc = make_staging(dom, st)) return rc;
-        HTRY(launch_unpack_f16(dom->buf[dom->cur], st.p, dom->fstride, n, dom->stream));
-        m = &st.view;
-    }
-    float *const *v = reinterpret_cast<float *const *>(m);
+    if (f16 && (field_mask & ~0x3u))
+        if (int rc = make_staging(dom, st, 1)) return rc;
     int k = 0;
-    for (int f = 0; f < 11; ++f)
-        if (field_mask & (1u << f))
-            HTRY(hipMemcpyAsync(v[f], host + (k++) * n, sizeof(float) * n, hipMemcpyHostToDevice, dom->stream));
-    if (f16) HTRY(launch_pack_f16(st.p, dom->buf[dom->cur], dom->fstride, n, dom->stream));
+    for (int f = 0; f < 11; ++f) {
+        if (!(field_mask & (1u << f))) continue;
+        const float *src = host + (k++) * n;
+        if (f16 && f >= 2) {
+            HTRY(hipMemcpyAsync(st.p, src, sizeof(float) * n, hipMemcpyHostToDevice, dom->stream));
+            HTRY(launch_pack_f16(st.p, dom->fstride, dom->buf[dom->cur], dom->fstride, n, f, f + 1, dom->stream));
+        } else {
+            HTRY(hipMemcpyAsync(dom->buf[dom->cur] + f * dom->fstride, src, sizeof(float) * n, hipMemcpyHostToDevice,
+                                dom->stream));
+        }
+    }
     HTRY(hipStreamSynchronize(dom->stream));
     return CGLBM_OK;
 }

@@ -30,3 +30,4 @@ struct cglbm_domain {
 
 // Tells the owning batch that a member's step count or current buffer changed outside it.
 void batch_member_changed(cglbm_batch *b);
+hipStream_t batch_stream(cglbm_batch *b);

@@ -77,9 +77,12 @@ hipError_t launch_march_q27_f32(const MarchArgs &a, MarchShape m, hipStream_t s)
 hipError_t launch_march_q27_f16(const MarchArgs &a, MarchShape m, hipStream_t s);
 hipError_t launch_march(const StepArgs &a, int q, hipStream_t s);
 
-// FP16-deviation states (storage.hpp) <-> FP32 compact fields (11 x fs floats, field f at f fs).
-hipError_t launch_pack_f16(const float *src, float *state, int64_t fs, int64_t n, hipStream_t s);
-hipError_t launch_unpack_f16(const float *state, float *dst, int64_t fs, int64_t n, hipStream_t s);
+// FP16-deviation states (storage.hpp, field stride fs) <-> FP32 compact fields [f0, f1), field f of
+// the FP32 side at (f - f0) * xs floats; n cells.
+hipError_t launch_pack_f16(const float *src, int64_t xs, float *state, int64_t fs, int64_t n, int f0, int f1,
+                           hipStream_t s);
+hipError_t launch_unpack_f16(const float *state, int64_t fs, float *dst, int64_t xs, int64_t n, int f0, int f1,
+                             hipStream_t s);
 
 // FP64 diagnostics: partial sums per block into `partials` (capacity kReduceBlocks * 6), then the
 // fixed-order final sum into `result` (6 doubles).

@@ -80,3 +80,32 @@ def test_domain_create_without_gpu_fails_cleanly():
         pass
     with pytest.raises(cg.CglbmError):
         cg.Domain(cg.Config(nx=8, ny=8, nz=8))
+    with pytest.raises(cg.CglbmError, match="HIP device"):
+        cg.Server()
+
+
+def test_server_struct_layout_matches_header():
+    """The ctypes mirrors of cglbm_request / cglbm_result / cglbm_server_opts / cglbm_server_stats
+    have the sizes the C compiler gives the header's structs (checked with the host compiler)."""
+    import ctypes
+    import subprocess
+    import tempfile
+    cg = load_product()
+    src = r'''
+#include <stdio.h>
+#include "cglbm.h"
+int main(void) {
+    printf("%zu %zu %zu %zu %zu\n", sizeof(cglbm_config), sizeof(cglbm_request), sizeof(cglbm_result),
+           sizeof(cglbm_server_opts), sizeof(cglbm_server_stats));
+    return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as tmp:
+        c = os.path.join(tmp, "sizes.c")
+        with open(c, "w") as f:
+            f.write(src)
+        exe = os.path.join(tmp, "sizes")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    want = [ctypes.sizeof(t) for t in (cg._Config, cg._Request, cg._Result, cg._ServerOpts, cg._ServerStats)]
+    assert got == want
