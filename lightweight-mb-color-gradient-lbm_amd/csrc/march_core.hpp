@@ -208,9 +208,27 @@ __device__ __forceinline__ float cdot(float vx, float vy, float vz) {
 struct Iface {
     bool on;     // interface indicator 4 rr rb / rho^2 > 1e-4 (lbm.cuh:181-187)
     bool small;  // |F| <= 1e-4
-    bool omega;  // Omega2 applied (false only in the conservative variant with small |F|)
+    bool omega;  // Omega2 applied (false in the conservative variant with small |F|, and for a
+                 // non-interface lane that runs the interface form with its wave)
     float Fx, Fy, Fz, af, af2, ha, coeff;
+    float rinv_af2, rinv_af;  // RN(1 / |F|^2), RN(1 / |F|): shared by the per-direction divisions
 };
+
+// a / b correctly rounded, given y = RN(1 / b) shared by many divisions by b. Markstein's
+// correction: with q = RN(a y) and the exact remainder r = a - b q (one fma), RN(q + r y) is the
+// correctly rounded quotient provided r does not underflow. Zero numerators return a itself
+// (a / b for finite nonzero b, sign included); numerators below 2^-100, where the remainder could
+// underflow, take the IEEE division. A NaN y (b NaN) keeps the NaN. Three operations instead of
+// the ten of the IEEE division sequence; bitwise equal to a / b (checked against the IEEE
+// quotient for every significand of b and the operand ranges of the colour-gradient terms).
+__device__ __forceinline__ float div_shared(float a, float b, float y) {
+    float q = a * y;
+    const float r = fmaf(-q, b, a);
+    q = fmaf(r, y, q);
+    q = (a == 0.0f && y == y) ? a : q;
+    if (fabsf(a) < 0x1p-100f && a != 0.0f) q = a / b;
+    return q;
+}
 
 __device__ __forceinline__ Iface iface_begin(const Mom &s) {
     Iface f;
@@ -220,9 +238,13 @@ __device__ __forceinline__ Iface iface_begin(const Mom &s) {
     f.small = false;
     f.omega = true;
     f.Fx = f.Fy = f.Fz = f.af = f.af2 = f.ha = f.coeff = 0.0f;
+    f.rinv_af2 = f.rinv_af = 0.0f;
     return f;
 }
 
+// Q == 19 also prepares the shared reciprocals and the per-lane mask of the wave-uniform
+// interface form; D3Q27 keeps the per-direction IEEE divisions (its register budget is tighter).
+template <int Q>
 __device__ __forceinline__ void iface_finish(Iface &f, const Mom &s, const Coeffs &c, int conservative) {
     f.af = sqrtf(f.Fx * f.Fx + f.Fy * f.Fy + f.Fz * f.Fz);
     f.af2 = f.af * f.af;
@@ -231,6 +253,11 @@ __device__ __forceinline__ void iface_finish(Iface &f, const Mom &s, const Coeff
     f.ha = 0.5f * c.Acoef * f.af;
     const float rho = s.rr + s.rb;
     f.coeff = c.beta * (s.rr * s.rb) / (rho * rho);
+    if constexpr (Q == 19) {
+        f.rinv_af2 = 1.0f / f.af2;
+        f.rinv_af = 1.0f / f.af;
+        f.omega = f.omega && f.on;
+    }
 }
 
 // Colour gradient F = (1/cs2) sum_{j>=1} w_j psi(x + c_j) c_j (forces.cuh:20-60) with psi read
@@ -337,7 +364,7 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
                 gi = feqi + c.oms * gneqi;
                 go = feqo + c.oms * gneqo;
             }
-            if constexpr (IFACE) {
+            if constexpr (IFACE && Q != 19) {
                 const float Fc = cdot<I>(fi.Fx, fi.Fy, fi.Fz);
                 const float c2q = (Fc * Fc) / fi.af2;
                 const float c2 = fi.small ? 0.0f : c2q;
@@ -351,6 +378,22 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
                 const float D = fi.coeff * cphi * wr;
                 sink(std::integral_constant<int, I>{}, aR * gi + D, aB * gi - D);
                 sink(std::integral_constant<int, O>{}, aR * go - D, aB * go + D);
+            } else if constexpr (IFACE) {
+                const float Fc = cdot<I>(fi.Fx, fi.Fy, fi.Fz);
+                const float c2q = div_shared(Fc * Fc, fi.af2, fi.rinv_af2);
+                const float c2 = fi.small ? 0.0f : c2q;
+                const float om = fi.ha * (w * c2 - bcoef<Q>(I));
+                if (fi.omega) {
+                    gi += om;
+                    go += om;
+                }
+                const float cq = div_shared(Fc * invcnorm(I), fi.af, fi.rinv_af);
+                const float cphi = fi.small ? 0.0f : cq;
+                const float D = fi.coeff * cphi * wr;
+                const float ri = aR * gi, bi = aB * gi, ro = aR * go, bo = aB * go;
+                // a non-interface lane running with an interface wave adds no recolouring term
+                sink(std::integral_constant<int, I>{}, fi.on ? ri + D : ri, fi.on ? bi - D : bi);
+                sink(std::integral_constant<int, O>{}, fi.on ? ro - D : ro, fi.on ? bo + D : bo);
             } else {
                 sink(std::integral_constant<int, I>{}, aR * gi, aB * gi);
                 sink(std::integral_constant<int, O>{}, aR * go, aB * go);

@@ -395,9 +395,14 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
                     emit_ghost<Q, M, R>(cur, curF, c, sink);
                 } else {
                     Iface f = iface_begin(cur);
-                    if (f.on) {
+                    // D3Q19: wave-uniform choice; a wave with any interface source runs the
+                    // interface form on every lane (non-interface lanes add no Omega2 or
+                    // recolouring term) instead of both forms under divergent masks. D3Q27
+                    // (longer interface form, higher register pressure) branches per lane.
+                    const bool iface = Q == 19 ? (__builtin_amdgcn_ballot_w64(f.on) != 0) : f.on;
+                    if (iface) {
                         colour_gradient<Q>(psi_reader(p), f.Fx, f.Fy, f.Fz);
-                        iface_finish(f, cur, c, a.conservative);
+                        iface_finish<Q>(f, cur, c, a.conservative);
                         emit_regular<Q, M, R, FORCED, true>(cur, c, f, sink);
                     } else {
                         emit_regular<Q, M, R, FORCED, false>(cur, c, f, sink);
