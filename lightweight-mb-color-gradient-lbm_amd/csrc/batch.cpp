@@ -45,6 +45,11 @@ struct cglbm_batch {
     int bundle = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // side streams: the kernel classes of one step run concurrently (fork from `stream`, join back)
+    static constexpr int kSide = 3;
+    hipStream_t side[kSide] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kSide] = {nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -158,9 +163,40 @@ int rebuild(cglbm_batch *b) {
 }
 
 // Enqueue step k of a bundle for every class (no host synchronisation).
+int ensure_side(cglbm_batch *b) {
+    if (b->fork) return CGLBM_OK;
+    for (int i = 0; i < cglbm_batch::kSide; ++i) {
+        HTRY(hipStreamCreate(&b->side[i]));
+        HTRY(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming));
+    }
+    HTRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    return CGLBM_OK;
+}
+
+// Enqueue step k of a bundle for every class (no host synchronisation). With several classes the
+// first runs on the batch stream and the others on side streams forked from it and joined back, so
+// small classes do not leave the GPU idle while they run; graph capture records the fork and join
+// as dependencies.
 int enqueue_step(cglbm_batch *b, int k) {
+    int nclass = 0;
+    for (const Class &c : b->classes) nclass += c.nitems > 0 ? 1 : 0;
+    const bool fan = nclass > 1;
+    bool used[cglbm_batch::kSide] = {false, false, false};
+    if (fan) {
+        if (int rc = ensure_side(b)) return rc;
+        HTRY(hipEventRecord(b->fork, b->stream));
+    }
+    int slot = 0;
     for (const Class &c : b->classes) {
         if (c.nitems == 0) continue;
+        hipStream_t s = b->stream;
+        if (fan && slot > 0) {
+            const int j = (slot - 1) % cglbm_batch::kSide;
+            s = b->side[j];
+            if (!used[j]) HTRY(hipStreamWaitEvent(s, b->fork, 0));
+            used[j] = true;
+        }
+        ++slot;
         BatchLaunch L{};
         L.descs = b->d_descs + c.desc_off;
         L.items = b->d_items + c.item_off;
@@ -172,8 +208,13 @@ int enqueue_step(cglbm_batch *b, int k) {
         L.forced = c.forced;
         L.store = c.store;
         L.shape = class_shape(b, c);
-        const hipError_t e = launch_batch(L, c.q, b->stream);
+        const hipError_t e = launch_batch(L, c.q, s);
         if (e != hipSuccess) return herr(e, "batch launch");
+    }
+    for (int j = 0; j < cglbm_batch::kSide; ++j) {
+        if (!used[j]) continue;
+        HTRY(hipEventRecord(b->join[j], b->side[j]));
+        HTRY(hipStreamWaitEvent(b->stream, b->join[j], 0));
     }
     return CGLBM_OK;
 }
@@ -254,6 +295,12 @@ int cglbm_batch_destroy(cglbm_batch *b) {
     if (b->d_base) (void)hipFree(b->d_base);
     (void)hipEventDestroy(b->ev0);
     (void)hipEventDestroy(b->ev1);
+    for (int i = 0; i < cglbm_batch::kSide; ++i) {
+        if (b->side[i]) (void)hipStreamSynchronize(b->side[i]);
+        if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
+        if (b->join[i]) (void)hipEventDestroy(b->join[i]);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
     (void)hipStreamDestroy(b->stream);
     delete b;
     return CGLBM_OK;
