@@ -119,7 +119,7 @@ def measure_batched(cg, steps: int, seq_steps: int, bundle: int = 10):
     b = cg.Batch()
     for d in doms:
         b.add(d)
-    b.autotune(1)
+    b.autotune(2)
     b.set_graph(bundle)
     b.step(2 * bundle)
     b.sync()

@@ -40,7 +40,6 @@ struct cglbm_batch {
     std::vector<Class> classes;
     bool dirty = true;
     MarchShape forced_shape{0, 0, 0};  // ty == 0: per-class autotuned/default
-    int grid_per_class = 0;             // 0: one block per work item
     // graph bundles
     int bundle = 0;
     hipGraph_t graph = nullptr;
@@ -203,7 +202,7 @@ int enqueue_step(cglbm_batch *b, int k) {
         L.nitems = c.nitems;
         L.base = b->d_base;
         L.k = k;
-        L.grid = b->grid_per_class > 0 ? std::min(b->grid_per_class, (c.nitems + 7) / 8 * 8) : (c.nitems + 7) / 8 * 8;
+        L.grid = (c.nitems + 7) / 8 * 8;  // one block per work item
         L.ybc = c.ybc;
         L.forced = c.forced;
         L.store = c.store;
@@ -436,7 +435,7 @@ int cglbm_batch_autotune(cglbm_batch *b, int32_t steps_per_candidate) {
     b->forced_shape = MarchShape{0, 0, 0};
     if (int rc = rebuild(b)) return rc;
     const int tys[3] = {4, 8, 16};
-    const int Ls[7] = {4, 8, 12, 16, 24, 32, 64};
+    const int Ls[8] = {4, 8, 12, 16, 24, 32, 48, 64};
     for (size_t ci = 0; ci < b->classes.size(); ++ci) {
         MarchShape best{0, 0, 0};
         double best_ms = 1e30;

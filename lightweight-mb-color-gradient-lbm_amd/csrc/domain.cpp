@@ -229,7 +229,7 @@ int cglbm_domain_autotune(cglbm_domain *dom, int32_t steps_per_candidate, int32_
     if (int rc = wait_batch(dom)) return rc;
     if (steps_per_candidate < 1 || steps_per_candidate > 1000) return set_error(CGLBM_EINVAL, "bad step count");
     const int tys[3] = {4, 8, 16};
-    const int Ls[9] = {2, 4, 6, 8, 12, 16, 24, 32, 64};
+    const int Ls[10] = {2, 4, 6, 8, 12, 16, 24, 32, 48, 64};
     MarchShape best{0, 0, 0};
     double best_ms = 1e30;
     const int nxt = 1 - dom->cur;

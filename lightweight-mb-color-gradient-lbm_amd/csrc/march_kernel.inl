@@ -471,8 +471,8 @@ __global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
 }
 
 // Batched form: many domains of one kernel class in one launch. Each work item names a domain
-// descriptor and a tile/chunk of it; blocks take items in XCD-contiguous ranges and loop while
-// items remain, so any grid size covers the list. The step counter (parity of the ping-pong and
+// descriptor and a tile/chunk of it; the grid has one block per item (rounded up to a multiple of
+// 8) and blocks take items in XCD-contiguous ranges. The step counter (parity of the ping-pong and
 // the inlet perturbation key) is base[0] + k, base living in device memory so a captured
 // multi-step graph can be replayed.
 template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST>
@@ -480,21 +480,20 @@ __global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
     march_batch_kernel(const BatchDesc *__restrict__ descs, const int4 *__restrict__ items, int nitems,
                        const int64_t *__restrict__ base, int k) {
     __shared__ Shared<Q, TY, NBUF> sh;
-    const int64_t kk = base[0] + k;
     const int nper = (nitems + 7) / 8;
-    for (int b = blockIdx.x; b < nper * 8; b += gridDim.x) {
-        const int logical = (b % 8) * nper + b / 8;
-        if (logical >= nitems) continue;
-        const int4 it = items[logical];
-        const BatchDesc &d = descs[it.x];
-        MarchArgs a = d.args;
-        const int64_t steps_done = d.step0 + kk;  // member steps completed before this launch
-        const int parity = d.cur0 ^ static_cast<int>(kk & 1);
-        a.A0 = parity ? d.buf1 : d.buf0;
-        a.B0 = parity ? d.buf0 : d.buf1;
-        a.step = steps_done + 1;
-        march_tile<Q, YBC, TY, NBUF, FORCED, ST>(a, sh, it.y, it.z, it.w, d.nchunk);
-    }
+    const int b = blockIdx.x;
+    const int logical = (b % 8) * nper + b / 8;
+    if (b >= nper * 8 || logical >= nitems) return;
+    const int4 it = items[logical];
+    const BatchDesc &d = descs[it.x];
+    const int64_t kk = base[0] + k;
+    MarchArgs a = d.args;
+    const int64_t steps_done = d.step0 + kk;  // member steps completed before this launch
+    const int parity = d.cur0 ^ static_cast<int>(kk & 1);
+    a.A0 = parity ? d.buf1 : d.buf0;
+    a.B0 = parity ? d.buf0 : d.buf1;
+    a.step = steps_done + 1;
+    march_tile<Q, YBC, TY, NBUF, FORCED, ST>(a, sh, it.y, it.z, it.w, d.nchunk);
 }
 
 template <int Q, int YBC, int TY, int NBUF, int ST>
