@@ -125,17 +125,28 @@ def measure_batched(cg, steps: int, seq_steps: int, bundle: int = 10):
     b.sync()
     ms = b.time_steps(steps)
     ms_seq = b.time_sequential(seq_steps)
+    # the reference's launch style: one thread per cell (the gather kernel), member after member
+    for d in doms:
+        d.set_kernel(cg.KERNEL_GATHER)
+    ms_ref = b.time_sequential(seq_steps)
+    for d in doms:
+        d.set_kernel(cg.KERNEL_AUTO)
     for d in doms:
         b.remove(d)
         d.close()
     b.close()
     mlups = cells * steps / (ms / 1e3) / 1e6
     mlups_seq = cells * seq_steps / (ms_seq / 1e3) / 1e6
+    mlups_ref = cells * seq_steps / (ms_ref / 1e3) / 1e6
     return {"domains": len(work), "cells": cells, "steps": steps, "ms_per_step": round(ms / steps, 4),
             "value": round(mlups, 1), "unit": "MLUPS", "graph_bundle": bundle,
-            "sequential_plain_launches": {"steps": seq_steps, "ms_per_step": round(ms_seq / seq_steps, 4),
-                                          "value": round(mlups_seq, 1)},
+            "sequential_plain_launches": {"kernel": "z-marching, size-aware default tile", "steps": seq_steps,
+                                          "ms_per_step": round(ms_seq / seq_steps, 4), "value": round(mlups_seq, 1)},
+            "sequential_reference_shaped_launches": {"kernel": "one thread per cell (gather)", "steps": seq_steps,
+                                                     "ms_per_step": round(ms_ref / seq_steps, 4),
+                                                     "value": round(mlups_ref, 1)},
             "speedup_vs_sequential": round(mlups / mlups_seq, 2),
+            "speedup_vs_reference_shaped": round(mlups / mlups_ref, 2),
             "achieved_gbs": round(BYTES_PER_UPDATE_F32 * cells / (ms / steps / 1e3) / 1e9, 1)}
 
 
@@ -222,7 +233,8 @@ def main() -> int:
     if not args.no_batched:
         print("bench: 256 batched domains", file=sys.stderr, flush=True)
         batched = measure_batched(cg, args.batched_steps, 10)
-        print(f"bench: batched {batched['value']:.0f} MLUPS, {batched['speedup_vs_sequential']}x sequential",
+        print(f"bench: batched {batched['value']:.0f} MLUPS, {batched['speedup_vs_sequential']}x sequential, "
+              f"{batched['speedup_vs_reference_shaped']}x reference-shaped sequential",
               file=sys.stderr, flush=True)
     serving = None
     if not args.no_serve:

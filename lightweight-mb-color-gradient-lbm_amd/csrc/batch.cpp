@@ -74,7 +74,7 @@ void drop_graph(cglbm_batch *b) {
 MarchShape class_shape(const cglbm_batch *b, const Class &c) {
     if (b->forced_shape.ty > 0) return b->forced_shape;
     if (c.shape_set) return c.shape;
-    return MarchShape{8, 16, c.q == 19 ? 2 : 1};
+    return MarchShape{c.q == 19 ? 16 : 4, 16, 1};  // the single-domain defaults
 }
 
 // Rebuild descriptors and work items from the current member states (synchronous).

@@ -16,8 +16,30 @@ int env_int(const char *name, int lo, int hi) {
 }  // namespace
 
 MarchShape march_default_shape(const StepArgs &a, int q) {
-    // CGLBM_MARCH_TY / CGLBM_MARCH_L / CGLBM_MARCH_NBUF override the defaults for tuning sweeps.
-    MarchShape m{8, 16, q == 19 ? 2 : 1};
+    // Untuned default, size-aware: the first shape, in order of per-cell efficiency measured by the
+    // tile sweeps on MI355X, that gives every CU a work-group (256); if none does (small domains),
+    // the one with the most work-groups. D3Q27 prefers 4-row tiles (its 16-row tile spills).
+    // CGLBM_MARCH_TY / CGLBM_MARCH_L / CGLBM_MARCH_NBUF override the choice for tuning sweeps.
+    static constexpr int kCand19[8][2] = {{16, 16}, {4, 16}, {16, 8}, {4, 8}, {16, 4}, {4, 4}, {4, 2}, {4, 1}};
+    static constexpr int kCand27[8][2] = {{4, 16}, {16, 16}, {4, 8}, {16, 8}, {4, 4}, {16, 4}, {4, 2}, {4, 1}};
+    constexpr int64_t kFill = 256;
+    const auto &cand = q == 19 ? kCand19 : kCand27;
+    MarchShape m{cand[0][0], cand[0][1], 1};
+    int64_t best_blocks = -1;
+    for (const auto &c : cand) {
+        const int L = c[1] < a.nz ? c[1] : a.nz;
+        const int64_t blocks =
+            int64_t((a.nx + 31) / 32) * ((a.ny + c[0] - 1) / c[0]) * ((a.nz + L - 1) / L);
+        if (blocks >= kFill) {
+            m = MarchShape{c[0], L, 1};
+            best_blocks = -2;
+            break;
+        }
+        if (blocks > best_blocks) {
+            best_blocks = blocks;
+            m = MarchShape{c[0], L, 1};
+        }
+    }
     if (const int ty = env_int("CGLBM_MARCH_TY", 1, 64)) m.ty = ty;
     if (const int L = env_int("CGLBM_MARCH_L", 1, 4096)) m.L = L;
     if (const int nb = env_int("CGLBM_MARCH_NBUF", 1, 2)) m.nbuf = nb;
