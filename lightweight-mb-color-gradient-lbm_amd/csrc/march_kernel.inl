@@ -61,10 +61,17 @@ __host__ __device__ constexpr int nslots() {
 }
 
 // The exchange planes carry a one-cell margin so a tile source can push every direction
-// without a bounds test (pushes that leave the tile land in the margin and are never read).
+// without a bounds test (pushes that leave the tile land in the margin and are never read) --
+// except the 16-row D3Q19 tile, where dropping the margin (pushes bounds-tested instead) is what
+// lets two exchange buffers fit the 160 KB LDS, so each plane needs one barrier instead of two.
+template <int Q, int TY>
+__host__ __device__ constexpr int margin() {
+    return (Q == 19 && TY == 16) ? 0 : 1;
+}
 template <int Q, int TY, int NBUF>
 struct Shared {
-    float2 xfer[NBUF][nslots<Q>()][TY + 2][TX + 2];
+    static constexpr int M = margin<Q, TY>();
+    float2 xfer[NBUF][nslots<Q>()][TY + 2 * M][TX + 2 * M];
     float psi[kPsiSlots][TY + 4][TX + 4];
 };
 
@@ -368,18 +375,23 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
 
         // ---- sources ----------------------------------------------------------------------
         if (srow >= 0) {
+            constexpr int MG = margin<Q, TY>();
             auto lds_push = [&](auto I, float fr, float fb) {
                 constexpr int i = decltype(I)::value;
                 if constexpr (cross_col(i)) {
                     const int txx = sx + kCx[i], tyy = sy + kCy[i];
-                    if (txx >= -1 && txx <= TX && tyy >= -1 && tyy <= TY)
-                        sh.xfer[xb][slot_of(i)][tyy + 1][txx + 1] = make_float2(fr, fb);
+                    if (txx >= -MG && txx < TX + MG && tyy >= -MG && tyy < TY + MG)
+                        sh.xfer[xb][slot_of(i)][tyy + MG][txx + MG] = make_float2(fr, fb);
                 }
             };
             auto own_sink = [&](auto I, float fr, float fb) {
                 constexpr int i = decltype(I)::value;
                 if constexpr (cross_col(i)) {
-                    sh.xfer[xb][slot_of(i)][sy + kCy[i] + 1][sx + kCx[i] + 1] = make_float2(fr, fb);
+                    constexpr int cx = kCx[i], cy = kCy[i];
+                    // own lanes sit inside the tile: only the sides the direction points to are tested
+                    const bool in = MG != 0 || ((cx >= 0 || sx > 0) && (cx <= 0 || sx < TX - 1) &&
+                                                (cy >= 0 || sy > 0) && (cy <= 0 || sy < TY - 1));
+                    if (in) sh.xfer[xb][slot_of(i)][sy + cy + MG][sx + cx + MG] = make_float2(fr, fb);
                 } else if constexpr (kCz[i] == 0) {
                     HB.v[i] = make_float2(fr, fb);
                 } else if constexpr (kCz[i] > 0) {
@@ -433,7 +445,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
             unroll<0, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
                 if constexpr (cross_col(i)) {
-                    const float2 v = sh.xfer[xb][slot_of(i)][sy + 1][sx + 1];
+                    const float2 v = sh.xfer[xb][slot_of(i)][sy + margin<Q, TY>()][sx + margin<Q, TY>()];
                     if constexpr (kCz[i] == 0) HB.v[i] = v;
                     else if constexpr (kCz[i] > 0) HC.v[i] = v;
                     else HA.v[i] = v;
