@@ -64,6 +64,31 @@ __host__ __device__ constexpr int nslots() {
 // without a bounds test (pushes that leave the tile land in the margin and are never read) --
 // except the 16-row D3Q19 tile, where dropping the margin (pushes bounds-tested instead) is what
 // lets two exchange buffers fit the 160 KB LDS, so each plane needs one barrier instead of two.
+// Ring waves per work-group. The 16-row tile runs four (8 own waves + 4 = 3 per SIMD): the row
+// and the column ring each split their direction pairs over two waves, so the halo work spreads
+// over all four SIMDs instead of loading two of them. Smaller tiles run two (rows, columns).
+template <int TY>
+__host__ __device__ constexpr int ring_waves() {
+    return TY == 16 ? 4 : 2;
+}
+template <int TY>
+__host__ __device__ constexpr int block_threads() {
+    return TX * TY + 64 * ring_waves<TY>();
+}
+// Half h (0 or 1) of the pairs in mask m: the first ceil(n/2) set bits, or the rest.
+__host__ __device__ constexpr uint32_t mask_half(uint32_t m, int h) {
+    int n = 0;
+    for (int k = 0; k < 32; ++k) n += (m >> k) & 1u;
+    uint32_t lo = 0;
+    int c = 0;
+    for (int k = 0; k < 32; ++k)
+        if ((m >> k) & 1u) {
+            if (c < (n + 1) / 2) lo |= 1u << k;
+            ++c;
+        }
+    return h == 0 ? lo : (m & ~lo);
+}
+
 template <int Q, int TY>
 __host__ __device__ constexpr int margin() {
     return (Q == 19 && TY == 16) ? 0 : 1;
@@ -213,8 +238,10 @@ template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST>
 __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBUF> &sh, int tix, int tiy, int chunk,
                                            int nchunk) {
     constexpr int NOWN = TX * TY;
+    constexpr int RW = ring_waves<TY>();
+    constexpr int NRING = 64 * RW;
     constexpr int PX = TX + 4, PY = TY + 4, NPSI = PX * PY;
-    constexpr int NPASS = (NPSI + 127) / 128;  // psi cells per ring lane
+    constexpr int NPASS = (NPSI + NRING - 1) / NRING;  // psi cells per ring lane
     constexpr uint32_t ALL = pair_mask_all<Q>();
     constexpr uint32_t MX = pair_mask_where<Q>(0), MY = pair_mask_where<Q>(1), MZ = pair_mask_where<Q>(2);
 
@@ -229,18 +256,23 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     const FieldBuf<ST> Bbuf(a.B0, a.fsB, a.spanB);
 
     // ---- role and source cell of this lane -----------------------------------------------
+    // ring waves: RW == 2 -> rows, columns; RW == 4 -> rows (pair half 0), rows (half 1),
+    // columns (half 0), columns (half 1)
     int role, sx, sy;
+    const int rw = t >= NOWN ? (t - NOWN) / 64 : 0;
+    const int half = RW == 4 ? (rw & 1) : 0;
+    const bool rows_wave = RW == 4 ? rw < 2 : rw == 0;
     if (t < NOWN) {
         role = kRoleOwn;
         sx = t % TX;
         sy = t / TX;
-    } else if (t < NOWN + 64) {
-        const int l = t - NOWN;
+    } else if (rows_wave) {
+        const int l = (t - NOWN) % 64;
         role = kRoleRows;
         sx = l % TX;
         sy = l < TX ? -1 : TY;
     } else {
-        const int l = t - NOWN - 64;
+        const int l = (t - NOWN) % 64;
         role = kRoleCols;
         if (l < TY) {
             sx = -1;
@@ -284,7 +316,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     float psi_r[NPASS], psi_b[NPASS];
 #pragma unroll
     for (int k = 0; k < NPASS; ++k) {
-        const int ci = (t - NOWN) + k * 128;
+        const int ci = (t - NOWN) + k * NRING;
         const int px = ci % PX - 2, py = ci / PX - 2;
         psi_x[k] = ((x0 + px) % nx + nx) % nx;
         const int rr_ = r0 + py;
@@ -309,7 +341,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
 #pragma unroll
         for (int k = 0; k < NPASS; ++k) {
             if (psi_row[k] > -2) {
-                const int ci = (t - NOWN) + k * 128;
+                const int ci = (t - NOWN) + k * NRING;
                 const float v = psi_row[k] >= 0 ? (psi_r[k] - psi_b[k]) / (psi_r[k] + psi_b[k]) : 0.0f;
                 sh.psi[slot][ci / PX][ci % PX] = v;
             }
@@ -424,12 +456,30 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
             if (role == kRoleOwn) {
                 if (edge) run(std::integral_constant<uint32_t, MZ>{}, std::false_type{}, own_sink);
                 else run(std::integral_constant<uint32_t, ALL>{}, std::true_type{}, own_sink);
+            } else if (RW == 2) {
+                if (role == kRoleRows) {
+                    if (edge) run(std::integral_constant<uint32_t, MY & MZ>{}, std::false_type{}, lds_push);
+                    else run(std::integral_constant<uint32_t, MY>{}, std::false_type{}, lds_push);
+                } else {
+                    if (edge) run(std::integral_constant<uint32_t, MX & MZ>{}, std::false_type{}, lds_push);
+                    else run(std::integral_constant<uint32_t, MX>{}, std::false_type{}, lds_push);
+                }
             } else if (role == kRoleRows) {
-                if (edge) run(std::integral_constant<uint32_t, MY & MZ>{}, std::false_type{}, lds_push);
-                else run(std::integral_constant<uint32_t, MY>{}, std::false_type{}, lds_push);
+                if (half == 0) {
+                    if (edge) run(std::integral_constant<uint32_t, mask_half(MY & MZ, 0)>{}, std::false_type{}, lds_push);
+                    else run(std::integral_constant<uint32_t, mask_half(MY, 0)>{}, std::false_type{}, lds_push);
+                } else {
+                    if (edge) run(std::integral_constant<uint32_t, mask_half(MY & MZ, 1)>{}, std::false_type{}, lds_push);
+                    else run(std::integral_constant<uint32_t, mask_half(MY, 1)>{}, std::false_type{}, lds_push);
+                }
             } else {
-                if (edge) run(std::integral_constant<uint32_t, MX & MZ>{}, std::false_type{}, lds_push);
-                else run(std::integral_constant<uint32_t, MX>{}, std::false_type{}, lds_push);
+                if (half == 0) {
+                    if (edge) run(std::integral_constant<uint32_t, mask_half(MX & MZ, 0)>{}, std::false_type{}, lds_push);
+                    else run(std::integral_constant<uint32_t, mask_half(MX, 0)>{}, std::false_type{}, lds_push);
+                } else {
+                    if (edge) run(std::integral_constant<uint32_t, mask_half(MX & MZ, 1)>{}, std::false_type{}, lds_push);
+                    else run(std::integral_constant<uint32_t, mask_half(MX, 1)>{}, std::false_type{}, lds_push);
+                }
             }
         }
         __syncthreads();
@@ -469,7 +519,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
 }
 
 template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST>
-__global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
+__global__ void __launch_bounds__(block_threads<TY>()) CGLBM_MARCH_OCC
     march_kernel(MarchArgs a, int ntx, int nty, int nchunk, int nblocks) {
     __shared__ Shared<Q, TY, NBUF> sh;
     const int b = blockIdx.x;
@@ -488,7 +538,7 @@ __global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
 // the inlet perturbation key) is base[0] + k, base living in device memory so a captured
 // multi-step graph can be replayed.
 template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST>
-__global__ void __launch_bounds__(TX *TY + 128) CGLBM_MARCH_OCC
+__global__ void __launch_bounds__(block_threads<TY>()) CGLBM_MARCH_OCC
     march_batch_kernel(const BatchDesc *__restrict__ descs, const int4 *__restrict__ items, int nitems,
                        const int64_t *__restrict__ base, int k) {
     __shared__ Shared<Q, TY, NBUF> sh;
@@ -519,11 +569,11 @@ hipError_t launch_shape(const MarchArgs &a, int L, hipStream_t s) {
     // the gather path (march_supports).
     if constexpr (YBC != kYPeriodic) {
         if (a.forced) {
-            march_kernel<Q, YBC, TY, NBUF, true, ST><<<grid, TX * TY + 128, 0, s>>>(a, ntx, nty, nchunk, nblocks);
+            march_kernel<Q, YBC, TY, NBUF, true, ST><<<grid, block_threads<TY>(), 0, s>>>(a, ntx, nty, nchunk, nblocks);
             return hipGetLastError();
         }
     }
-    march_kernel<Q, YBC, TY, NBUF, false, ST><<<grid, TX * TY + 128, 0, s>>>(a, ntx, nty, nchunk, nblocks);
+    march_kernel<Q, YBC, TY, NBUF, false, ST><<<grid, block_threads<TY>(), 0, s>>>(a, ntx, nty, nchunk, nblocks);
     return hipGetLastError();
 }
 
@@ -556,7 +606,7 @@ hipError_t launch_march_qs(const MarchArgs &a, MarchShape m, hipStream_t s) {
 template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST>
 hipError_t launch_batch_shape(const BatchLaunch &L, hipStream_t s) {
     march_batch_kernel<Q, YBC, TY, NBUF, FORCED, ST>
-        <<<L.grid, TX * TY + 128, 0, s>>>(L.descs, L.items, L.nitems, L.base, L.k);
+        <<<L.grid, block_threads<TY>(), 0, s>>>(L.descs, L.items, L.nitems, L.base, L.k);
     return hipGetLastError();
 }
 
