@@ -74,10 +74,11 @@ void drop_graph(cglbm_batch *b) {
 MarchShape class_shape(const cglbm_batch *b, const Class &c) {
     if (b->forced_shape.ty > 0) return b->forced_shape;
     if (c.shape_set) return c.shape;
-    // Untuned class defaults, from class-by-class tile sweeps of batched members on MI355X: 4-row
-    // tiles (three work-groups per CU) everywhere, 32-plane chunks except for bounded y.
-    if (c.ybc != 0) return MarchShape{4, c.q == 19 ? 16 : 32, 1};
-    return MarchShape{4, 32, 1};
+    // Untuned class defaults, from class-by-class tile sweeps of batched members on MI355X:
+    // 16-row tiles for D3Q27 and periodic D3Q19, 4-row tiles for bounded-y D3Q19.
+    if (c.q == 27) return MarchShape{16, 16, 1};
+    if (c.ybc != 0) return MarchShape{4, 16, 1};
+    return MarchShape{16, 32, 1};
 }
 
 // Rebuild descriptors and work items from the current member states (synchronous).

@@ -17,13 +17,13 @@ int env_int(const char *name, int lo, int hi) {
 
 MarchShape march_default_shape(const StepArgs &a, int q) {
     // Untuned default, size-aware: the first shape, in order of per-cell efficiency measured by the
-    // tile sweeps on MI355X, that gives every CU a work-group (256); if none does (small domains),
-    // the one with the most work-groups. D3Q27 prefers 4-row tiles (its 16-row tile spills).
+    // tile sweeps on MI355X (both stencils), that gives every CU a work-group (256); if none does
+    // (small domains), the one with the most work-groups.
     // CGLBM_MARCH_TY / CGLBM_MARCH_L / CGLBM_MARCH_NBUF override the choice for tuning sweeps.
-    static constexpr int kCand19[8][2] = {{16, 16}, {4, 16}, {16, 8}, {4, 8}, {16, 4}, {4, 4}, {4, 2}, {4, 1}};
-    static constexpr int kCand27[8][2] = {{4, 16}, {16, 16}, {4, 8}, {16, 8}, {4, 4}, {16, 4}, {4, 2}, {4, 1}};
+    static constexpr int kCand[8][2] = {{16, 16}, {4, 16}, {16, 8}, {4, 8}, {16, 4}, {4, 4}, {4, 2}, {4, 1}};
     constexpr int64_t kFill = 256;
-    const auto &cand = q == 19 ? kCand19 : kCand27;
+    (void)q;
+    const auto &cand = kCand;
     MarchShape m{cand[0][0], cand[0][1], 1};
     int64_t best_blocks = -1;
     for (const auto &c : cand) {
