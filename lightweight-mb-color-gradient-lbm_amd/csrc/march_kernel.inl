@@ -8,11 +8,13 @@
 //    waves hold one source/target column per lane. Populations that stay in a column (rest, +z,
 //    -z) stay in registers; the others cross to the target column through LDS, one float2 per
 //    (direction, target).
-//  * Two ring waves build the halo sources around the tile, each only for the direction pairs
-//    that can enter it: the row wave (rows -1 and TY) the pairs with c_y != 0, the column wave
-//    (columns -1, 32 and the corners) the pairs with c_x != 0. The first and last source plane
-//    of a chunk build only the pairs with c_z != 0. The ring waves also form the phase field
-//    psi = (rr - rb)/(rr + rb) once per cell, two planes ahead, into an LDS ring (halo 2).
+//  * Ring waves build the halo sources around the tile, each only for the direction pairs that
+//    can enter it: the row ring (rows -1 and TY) the pairs with c_y != 0, the column ring
+//    (columns -1, 32 and the corners) the pairs with c_x != 0 (two waves; the 16-row tile splits
+//    each ring's pairs over two waves, four in all, so the halo work spreads over the four SIMDs).
+//    The first and last source plane of a chunk build only the pairs with c_z != 0. The ring
+//    waves also form the phase field psi = (rr - rb)/(rr + rb) once per cell, two planes ahead,
+//    into an LDS ring (halo 2).
 //  * Target plane t gets its +z, 0 and -z contributions at source planes t-1, t, t+1; the partial
 //    sets wait in registers and plane t is accumulated in the reference order i = 0..Q-1.
 //  * NBUF = 2 double-buffers the LDS exchange so each plane needs one barrier; NBUF = 1 uses
