@@ -109,3 +109,11 @@ int main(void) {
         got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     want = [ctypes.sizeof(t) for t in (cg._Config, cg._Request, cg._Result, cg._ServerOpts, cg._ServerStats)]
     assert got == want
+
+
+def test_fp16_state_over_4gib_is_rejected():
+    """FP16 states are addressed with 32-bit buffer offsets: a state of 4 GiB or more is refused at
+    creation, before any device call (so this runs without a GPU)."""
+    cg = load_product()
+    with pytest.raises(cg.CglbmError, match="4 GiB"):
+        cg.Domain(cg.Config(nx=560, ny=560, nz=560), storage=cg.STORE_F16)
