@@ -14,6 +14,7 @@
 #include "domain_internal.hpp"
 #include "kernels.hpp"
 #include "state.hpp"
+#include "storage.hpp"
 
 using namespace cglbm;
 
@@ -121,6 +122,8 @@ int rebuild(cglbm_batch *b) {
             const bool ok = d->storage == CGLBM_STORE_F16 ? march_args_f16(sa, d->buf[0], d->buf[1], d->fstride, bd.args)
                                                           : march_args(sa, bd.args);
             if (!ok) return set_error(CGLBM_EINVAL, "batch member not addressable by the fast path");
+            if (bd.args.store == kStoreF32Big)
+                return set_error(CGLBM_EINVAL, "batch members must stay below 4 GiB per state");
             bd.buf0 = d->buf[0];
             bd.buf1 = d->buf[1];
             bd.cur0 = d->cur;

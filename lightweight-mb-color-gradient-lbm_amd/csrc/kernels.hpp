@@ -75,6 +75,8 @@ hipError_t launch_march_q19_f32(const MarchArgs &a, MarchShape m, hipStream_t s)
 hipError_t launch_march_q19_f16(const MarchArgs &a, MarchShape m, hipStream_t s);
 hipError_t launch_march_q27_f32(const MarchArgs &a, MarchShape m, hipStream_t s);
 hipError_t launch_march_q27_f16(const MarchArgs &a, MarchShape m, hipStream_t s);
+hipError_t launch_march_q19_f32big(const MarchArgs &a, MarchShape m, hipStream_t s);
+hipError_t launch_march_q27_f32big(const MarchArgs &a, MarchShape m, hipStream_t s);
 hipError_t launch_march(const StepArgs &a, int q, hipStream_t s);
 
 // FP16-deviation states (storage.hpp, field stride fs) <-> FP32 compact fields [f0, f1), field f of

@@ -158,6 +158,34 @@ struct FieldBuf {
     }
 };
 
+// FP32 state of 4 GiB or more: the same accesses through a 64-bit pointer (the field base is
+// uniform, the cell offset a 32-bit VGPR, so the loads keep the scalar-base plus vector-offset
+// form). Every access of the kernel stays inside the state, as with the buffer form.
+template <>
+struct FieldBuf<kStoreF32Big> {
+    float *p;
+    uint64_t fs;
+    __device__ __forceinline__ FieldBuf(const float *base, uint32_t fs_, uint32_t)
+        : p(const_cast<float *>(base)), fs(fs_) {}
+    __device__ __forceinline__ float ld(int f, uint32_t id) const { return p[static_cast<uint64_t>(f) * fs + id]; }
+    __device__ __forceinline__ void st(int f, uint32_t id, float v) const { p[static_cast<uint64_t>(f) * fs + id] = v; }
+    __device__ __forceinline__ Mom mom(uint32_t id) const {
+        Mom m;
+        m.rr = ld(0, id);
+        m.rb = ld(1, id);
+        m.ux = ld(2, id);
+        m.uy = ld(3, id);
+        m.uz = ld(4, id);
+        m.pxx = ld(5, id);
+        m.pxy = ld(6, id);
+        m.pyy = ld(7, id);
+        m.pyz = ld(8, id);
+        m.pzz = ld(9, id);
+        m.pxz = ld(10, id);
+        return m;
+    }
+};
+
 // Boundary planes of the new state (jet/boundary_conditions.cuh:16-109 or the wall copies of
 // rti/rtiIn.cuh:77-119), written by the lane that has just finished row y = 1 (inlet/bottom) or
 // y = ny (outlet/top) of column (gx, zt).

@@ -56,13 +56,15 @@ bool march_args(const StepArgs &a, MarchArgs &m) {
     const int64_t nyt = a.ybc == kYPeriodic ? a.ny : a.ny + 2;
     const int64_t last = (a.nx - 1) + int64_t(a.py) * (nyt - 1) + int64_t(a.pz) * (a.nz - 1);
     const int64_t spanA = (fsA * 10 + last + 1) * 4, spanB = (fsB * 10 + last + 1) * 4;
-    if (spanA >= (int64_t(1) << 32) || spanB >= (int64_t(1) << 32)) return false;
+    // cell offsets and field strides stay 32-bit; a state of 4 GiB or more takes 64-bit pointers
+    if (last >= (int64_t(1) << 32) || fsA >= (int64_t(1) << 32) || fsB >= (int64_t(1) << 32)) return false;
+    const bool big = spanA >= (int64_t(1) << 32) || spanB >= (int64_t(1) << 32);
     m.A0 = a.A[0];
     m.B0 = a.B[0];
     m.fsA = static_cast<uint32_t>(fsA);
     m.fsB = static_cast<uint32_t>(fsB);
-    m.spanA = static_cast<uint32_t>(spanA);
-    m.spanB = static_cast<uint32_t>(spanB);
+    m.spanA = big ? 0u : static_cast<uint32_t>(spanA);
+    m.spanB = big ? 0u : static_cast<uint32_t>(spanB);
     m.nx = a.nx;
     m.ny = a.ny;
     m.nz = a.nz;
@@ -80,7 +82,7 @@ bool march_args(const StepArgs &a, MarchArgs &m) {
     m.jet_noise = a.jet_noise;
     m.seed = a.seed;
     m.step = a.step;
-    m.store = kStoreF32;
+    m.store = big ? kStoreF32Big : kStoreF32;
     return true;
 }
 
@@ -112,12 +114,18 @@ hipError_t launch_march_shape(const StepArgs &a, int q, MarchShape m, hipStream_
 
 hipError_t launch_march_args(const MarchArgs &a, int q, MarchShape m, hipStream_t s) {
     if ((m.ty != 4 && m.ty != 8 && m.ty != 16) || m.L < 1 || (m.nbuf != 1 && m.nbuf != 2)) return hipErrorInvalidValue;
+    if (a.store == kStoreF32Big) {
+        if (q == 19) return launch_march_q19_f32big(a, m, s);
+        if (q == 27) return launch_march_q27_f32big(a, m, s);
+        return hipErrorInvalidValue;
+    }
     if (q == 19) return a.store == kStoreF16 ? launch_march_q19_f16(a, m, s) : launch_march_q19_f32(a, m, s);
     if (q == 27) return a.store == kStoreF16 ? launch_march_q27_f16(a, m, s) : launch_march_q27_f32(a, m, s);
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_batch(const BatchLaunch &L, int q, hipStream_t s) {
+    if (L.store == kStoreF32Big) return hipErrorInvalidValue;  // batch members stay below 4 GiB per state
     if (q == 19) return L.store == kStoreF16 ? launch_batch_q19_f16(L, s) : launch_batch_q19_f32(L, s);
     if (q == 27) return L.store == kStoreF16 ? launch_batch_q27_f16(L, s) : launch_batch_q27_f32(L, s);
     return hipErrorInvalidValue;
