@@ -150,6 +150,44 @@ def measure_batched(cg, steps: int, seq_steps: int, bundle: int = 10):
             "achieved_gbs": round(BYTES_PER_UPDATE_F32 * cells / (ms / steps / 1e3) / 1e9, 1)}
 
 
+def measure_case(cg, cfg, init, steps: int, storage=None, warmup: int = 5):
+    """One configuration run for `steps` timed steps from its initial state (autotuned tile)."""
+    storage = cg.STORE_F32 if storage is None else storage
+    with cg.Domain(cfg, storage=storage) as d:
+        init(d)
+        m0 = d.reduce()
+        tile = d.autotune(2)[:3]
+        d.step(warmup)
+        d.sync()
+        ms = d.time_steps(steps)
+        m1 = d.reduce()
+    cells = cfg.interior_cells
+    # relative per-phase mass change (None where a phase starts empty, as the jet's injected one)
+    drift = [float((m1[k] - m0[k]) / m0[k]) if m0[k] > 0 else None for k in (0, 1)]
+    return {"value": round(cells * steps / (ms / 1e3) / 1e6, 1), "unit": "MLUPS", "steps": steps,
+            "ms_total": round(ms, 2), "tile": tile, "bad_cells": int(m1[5]), "mass_drift_rel": drift}
+
+
+def measure_configs(cg):
+    """The other workloads of the reference's configuration set, each at its stated size and length:
+    the 64^3 D3Q19 bubble (1000 steps), the D3Q27 round jet (128 x 256 x 128 with the jet axis y,
+    inlet radius 8, u 0.05 with 1% noise, zero-gradient outflow, 2000 steps) and the D3Q19
+    Rayleigh-Taylor case (128 x 512 x 128, walls at the ends of y, Atwood 0.5, g 1e-5, 5000 steps)
+    with FP16-deviation and FP32 storage."""
+    out = {}
+    c1 = cg.Config(nx=64, ny=64, nz=64, q=19, tau=0.8, sigma=0.01, beta=0.7)
+    out["bubble64_d3q19"] = measure_case(cg, c1, lambda d: d.init_bubble(radius=12.0, width=4.0), 1000)
+    c2 = cg.Config(nx=128, ny=256, nz=128, q=27, ybc=cg.Y_JET, outlet=cg.OUTLET_ZERO_GRADIENT, tau=0.55,
+                   sigma=0.005, beta=0.7, jet_radius=8.0, jet_velocity=0.05, jet_noise=0.01, seed=0)
+    out["jet_d3q27_128x256x128"] = measure_case(cg, c2, lambda d: d.init_jet(), 2000)
+    c3 = cg.Config(nx=128, ny=512, nz=128, q=19, ybc=cg.Y_WALLS, tau=0.6, sigma=0.002, beta=0.7, gravity=1e-5,
+                   atwood=0.5)
+    rti = lambda d: d.init_rti(2.0, 4.0, 8, 0.2, 0)
+    out["rti_d3q19_128x512x128_f16"] = measure_case(cg, c3, rti, 5000, storage=cg.STORE_F16)
+    out["rti_d3q19_128x512x128_f32"] = measure_case(cg, c3, rti, 5000)
+    return out
+
+
 def traffic_from_profiles(workload: str):
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(path):
@@ -206,6 +244,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-domain batched measurement")
     ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--no-serve", action="store_true", help="skip the Poisson serving measurement")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other configurations (bubble64, jet, RTI)")
     ap.add_argument("--serve-rate", type=float, default=6.0, help="serving arrivals per second")
     ap.add_argument("--serve-duration", type=float, default=60.0, help="serving arrival window (s)")
     ap.add_argument("--tile19", default=None, help="fixed D3Q19 tile 'rows,chunk,buffers' instead of autotuning")
@@ -236,6 +275,11 @@ def main() -> int:
         print(f"bench: batched {batched['value']:.0f} MLUPS, {batched['speedup_vs_sequential']}x sequential, "
               f"{batched['speedup_vs_reference_shaped']}x reference-shaped sequential",
               file=sys.stderr, flush=True)
+    configs = None
+    if not args.no_configs:
+        print("bench: other configurations (64^3 bubble, D3Q27 jet, RTI FP16/FP32)", file=sys.stderr, flush=True)
+        configs = measure_configs(cg)
+        print("bench: " + ", ".join(f"{k} {v['value']:.0f}" for k, v in configs.items()), file=sys.stderr, flush=True)
     serving = None
     if not args.no_serve:
         import cglbm.serving as sv
@@ -284,6 +328,7 @@ def main() -> int:
                            "d3q19_f16_storage": r16["mass_drift_rel"]},
         "batched": batched,
         "serving": serving,
+        "other_configs": configs,
     }
     print(json.dumps(line))
     return 0
