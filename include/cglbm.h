@@ -124,13 +124,23 @@ int64_t cglbm_domain_bytes(const cglbm_config *cfg, int32_t storage);
 int cglbm_domain_set_kernel(cglbm_domain *dom, int32_t kernel);
 /* Tile shape of the z-marching kernel: ty tile rows (4, 8 or 16; tiles are 32 cells wide),
  * L planes per z-chunk, nbuf LDS exchange buffers (2: one barrier per plane; 1: half the LDS,
- * two barriers). ty = 0 returns to automatic selection (autotuned record, else default). */
+ * two barriers). L = 0 asks for planned chunks: every 64 steps the step records which waves took
+ * the interface form and a planner cuts each column tile's z range into chunks of equal modelled
+ * cost, one work-group per CU (results are bitwise the same for every shape). ty = 0 returns to
+ * automatic selection (autotuned record, else 16-row tiles with planned chunks). */
 int cglbm_domain_set_tile(cglbm_domain *dom, int32_t ty, int32_t L, int32_t nbuf);
-/* Autotuner: times each candidate tile shape for steps_per_candidate updates from the current
- * state (the state does not advance), keeps the fastest for this domain and records it for every
- * later domain or launch of the same (q, ybc, nx, ny, nz). Outputs may be NULL. Synchronous. */
+/* Autotuner: times each candidate tile shape (planned chunks, L = 0, among them) for
+ * steps_per_candidate updates from the current state (the state does not advance), keeps the
+ * fastest for this domain and records it for every later domain or launch of the same
+ * (q, ybc, nx, ny, nz, storage, body force on/off); launches without a plan take the size-aware
+ * uniform chunk length for a recorded L = 0. Outputs may be NULL. Synchronous. */
 int cglbm_domain_autotune(cglbm_domain *dom, int32_t steps_per_candidate, int32_t *ty, int32_t *L,
                           int32_t *nbuf, double *ms_per_step);
+/* The domain's current chunk plan (for tiles with L = 0): *n_items work items, each four int32
+ * {tile x, tile y, z0, z1} (z0 == z1: an idle work-group), copied into items when cap >= *n_items;
+ * *ty = the plan's tile rows. *n_items = 0 when no plan has been made since the state was last
+ * set. Waits for the domain's stream. */
+int cglbm_domain_plan(cglbm_domain *dom, int32_t *items, int32_t cap, int32_t *n_items, int32_t *ty);
 int64_t cglbm_domain_step_count(const cglbm_domain *dom);
 void *cglbm_domain_stream(cglbm_domain *dom);
 

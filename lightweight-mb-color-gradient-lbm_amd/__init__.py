@@ -35,7 +35,7 @@ EXPORTS = (
     "cglbm_last_error", "cglbm_abi_version", "cglbm_compact_layout", "cglbm_launch_rcs", "cglbm_launch_ybc",
     "cglbm_domain_create", "cglbm_domain_destroy", "cglbm_domain_moments", "cglbm_domain_config",
     "cglbm_domain_bytes", "cglbm_domain_set_kernel", "cglbm_domain_set_tile", "cglbm_domain_autotune",
-    "cglbm_domain_step_count", "cglbm_domain_stream",
+    "cglbm_domain_step_count", "cglbm_domain_stream", "cglbm_domain_plan",
     "cglbm_init_bubble", "cglbm_init_jet", "cglbm_init_rti", "cglbm_step", "cglbm_time_steps", "cglbm_sync",
     "cglbm_copy_out", "cglbm_copy_in", "cglbm_reduce", "cglbm_write_vti", "cglbm_write_vti_host",
     "cglbm_batch_create", "cglbm_batch_destroy", "cglbm_batch_add", "cglbm_batch_remove", "cglbm_batch_size",
@@ -180,6 +180,8 @@ def lib():
         "cglbm_domain_bytes": ([C, ctypes.c_int32], ctypes.c_int64),
         "cglbm_domain_set_kernel": ([D, ctypes.c_int32], ctypes.c_int),
         "cglbm_domain_set_tile": ([D, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
+        "cglbm_domain_plan": ([D, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                               ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
         "cglbm_domain_autotune": ([D, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                    ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "cglbm_domain_step_count": ([D], ctypes.c_int64),
@@ -309,6 +311,17 @@ class Domain:
 
     def set_tile(self, ty: int, L: int = 0, nbuf: int = 2):
         _check(lib().cglbm_domain_set_tile(self._h, ty, L, nbuf), "cglbm_domain_set_tile")
+
+    def plan(self):
+        """(tile rows, items) of the current chunk plan; items is an (n, 4) int32 array of
+        {tile x, tile y, z0, z1} (z0 == z1: idle), empty when no plan exists."""
+        n, ty = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().cglbm_domain_plan(self._h, None, 0, ctypes.byref(n), ctypes.byref(ty)), "cglbm_domain_plan")
+        items = np.zeros((max(n.value, 0), 4), dtype=np.int32)
+        if n.value > 0:
+            _check(lib().cglbm_domain_plan(self._h, items.ctypes.data, n.value, ctypes.byref(n), ctypes.byref(ty)),
+                   "cglbm_domain_plan")
+        return ty.value, items
 
     def autotune(self, steps_per_candidate: int = 3):
         """Returns (tile rows, chunk length, exchange buffers, ms per step) of the fastest shape."""

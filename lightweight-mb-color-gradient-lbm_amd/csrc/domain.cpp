@@ -72,6 +72,8 @@ void release(cglbm_domain *d) {
         if (b) (void)hipFree(b);
     if (d->d_partials) (void)hipFree(d->d_partials);
     if (d->d_result) (void)hipFree(d->d_result);
+    if (d->plan.items) (void)hipFree(d->plan.items);
+    if (d->plan.wmap) (void)hipFree(d->plan.wmap);
     if (d->ev0) (void)hipEventDestroy(d->ev0);
     if (d->ev1) (void)hipEventDestroy(d->ev1);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -89,6 +91,7 @@ int apply_ybc_views(cglbm_domain *d, const cglbm_moments &m) {
 // assemble the FP32 state in staging and encode it once at the end.
 int upload_init(cglbm_domain *d, const std::vector<float> &host, bool bc) {
     const int64_t n = stored_cells(d->cfg);
+    d->plan.ty = 0;  // a new state: the next planned step records its interface map afresh
     Staging st;
     const cglbm_moments *m = &d->view[d->cur];
     if (d->storage == CGLBM_STORE_F16) {
@@ -124,7 +127,8 @@ extern "C" {
 int64_t cglbm_domain_bytes(const cglbm_config *cfg, int32_t storage) {
     if (validate_config(cfg)) return -1;
     if (storage != CGLBM_STORE_F32 && storage != CGLBM_STORE_F16) return -1;
-    return 2 * state_bytes(storage, field_stride(*cfg)) + static_cast<int64_t>(sizeof(double)) * (kReduceBlocks * 6 + 8);
+    return 2 * state_bytes(storage, field_stride(*cfg)) + static_cast<int64_t>(sizeof(double)) * (kReduceBlocks * 6 + 8) +
+           plan_device_bytes(*cfg);
 }
 
 // allocate_device_memory (memory.cuh:153-164): two zeroed 11-field states.
@@ -162,6 +166,12 @@ int cglbm_domain_create(const cglbm_config *cfg, int32_t storage, cglbm_domain *
     if (e == hipSuccess) e = hipEventCreate(&d->ev1);
     if (e == hipSuccess) e = hipMalloc(&d->d_partials, sizeof(double) * kReduceBlocks * 6);
     if (e == hipSuccess) e = hipMalloc(&d->d_result, sizeof(double) * 8);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&d->plan.ncu, hipDeviceAttributeMultiprocessorCount, d->device);
+    if (e == hipSuccess) {
+        d->plan.cap = plan_capacity(*cfg, d->plan.ncu);
+        e = hipMalloc(&d->plan.items, sizeof(int4) * d->plan.cap);
+    }
+    if (e == hipSuccess) e = hipMalloc(&d->plan.wmap, plan_wmap_bytes(*cfg));
     if (e == hipSuccess) e = hipMemsetAsync(d->buf[0], 0, bytes, d->stream);
     if (e == hipSuccess) e = hipMemsetAsync(d->buf[1], 0, bytes, d->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
@@ -214,9 +224,21 @@ int cglbm_domain_set_tile(cglbm_domain *dom, int32_t ty, int32_t L, int32_t nbuf
         dom->shape = MarchShape{0, 0, 0};
         return CGLBM_OK;
     }
-    if ((ty != 4 && ty != 8 && ty != 16) || L < 1 || L > dom->cfg.nz || (nbuf != 1 && nbuf != 2))
-        return set_error(CGLBM_EINVAL, "tile rows must be 4, 8 or 16, 1 <= L <= nz, nbuf 1 or 2");
+    if ((ty != 4 && ty != 8 && ty != 16) || L < 0 || L > dom->cfg.nz || (nbuf != 1 && nbuf != 2))
+        return set_error(CGLBM_EINVAL, "tile rows must be 4, 8 or 16, 0 <= L <= nz, nbuf 1 or 2");
     dom->shape = MarchShape{ty, L, nbuf};
+    return CGLBM_OK;
+}
+
+int cglbm_domain_plan(cglbm_domain *dom, int32_t *items, int32_t cap, int32_t *n_items, int32_t *ty) {
+    if (int rc = check_dom(dom)) return rc;
+    if (!n_items || !ty || cap < 0 || (cap > 0 && !items)) return set_error(CGLBM_EINVAL, "bad arguments");
+    HTRY(hipStreamSynchronize(dom->stream));
+    const MarchPlanState &p = dom->plan;
+    *ty = p.ty;
+    *n_items = p.ty > 0 ? plan_items(dom->cfg, p.ty, p.ncu) : 0;
+    if (*n_items > 0 && cap >= *n_items)
+        HTRY(hipMemcpy(items, p.items, sizeof(int4) * static_cast<size_t>(*n_items), hipMemcpyDeviceToHost));
     return CGLBM_OK;
 }
 
@@ -229,7 +251,7 @@ int cglbm_domain_autotune(cglbm_domain *dom, int32_t steps_per_candidate, int32_
     if (int rc = wait_batch(dom)) return rc;
     if (steps_per_candidate < 1 || steps_per_candidate > 1000) return set_error(CGLBM_EINVAL, "bad step count");
     const int tys[3] = {4, 8, 16};
-    const int Ls[10] = {2, 4, 6, 8, 12, 16, 24, 32, 48, 64};
+    const int Ls[11] = {0, 2, 4, 6, 8, 12, 16, 24, 32, 48, 64};  // 0: planned chunks
     MarchShape best{0, 0, 0};
     double best_ms = 1e30;
     const int nxt = 1 - dom->cur;
@@ -240,13 +262,13 @@ int cglbm_domain_autotune(cglbm_domain *dom, int32_t steps_per_candidate, int32_
             const MarchShape m{ty, L, nbuf};
             // warm-up launch, then the timed ones
             if (int rc = launch_step(&dom->view[dom->cur], &dom->view[nxt], &dom->cfg, &dom->layout, dom->steps + 1,
-                                     CGLBM_KERNEL_MARCH, &m, dom->stream, dom->storage, dom->fstride))
+                                     CGLBM_KERNEL_MARCH, &m, dom->stream, dom->storage, dom->fstride, &dom->plan))
                 return rc;
             HTRY(hipEventRecord(dom->ev0, dom->stream));
             for (int k = 0; k < steps_per_candidate; ++k)
                 if (int rc = launch_step(&dom->view[dom->cur], &dom->view[nxt], &dom->cfg, &dom->layout,
                                          dom->steps + 1, CGLBM_KERNEL_MARCH, &m, dom->stream, dom->storage,
-                                         dom->fstride))
+                                         dom->fstride, &dom->plan))
                     return rc;
             HTRY(hipEventRecord(dom->ev1, dom->stream));
             HTRY(hipEventSynchronize(dom->ev1));
@@ -259,7 +281,7 @@ int cglbm_domain_autotune(cglbm_domain *dom, int32_t steps_per_candidate, int32_
             }
         }
     dom->shape = best;
-    tuned_shape_store(dom->cfg, best);
+    tuned_shape_store(dom->cfg, dom->storage, best);
     if (ty_out) *ty_out = best.ty;
     if (L_out) *L_out = best.L;
     if (nbuf_out) *nbuf_out = best.nbuf;
@@ -370,7 +392,7 @@ int cglbm_step(cglbm_domain *dom, int32_t nsteps) {
     for (int k = 0; k < nsteps; ++k) {
         const int nxt = 1 - dom->cur;
         if (int rc = launch_step(&dom->view[dom->cur], &dom->view[nxt], &dom->cfg, &dom->layout, dom->steps + 1,
-                                 dom->kernel, &dom->shape, dom->stream, dom->storage, dom->fstride))
+                                 dom->kernel, &dom->shape, dom->stream, dom->storage, dom->fstride, &dom->plan))
             return rc;
         dom->cur = nxt;
         ++dom->steps;
@@ -429,6 +451,7 @@ int cglbm_copy_in(cglbm_domain *dom, uint32_t field_mask, const float *host) {
     if (!host || (field_mask & ~0x7FFu)) return set_error(CGLBM_EINVAL, "bad arguments");
     const int64_t n = stored_cells(dom->cfg);
     const bool f16 = dom->storage == CGLBM_STORE_F16;
+    dom->plan.ty = 0;
     Staging st;
     if (f16 && (field_mask & ~0x3u))
         if (int rc = make_staging(dom, st, 1)) return rc;

@@ -5,6 +5,7 @@
 
 #include "cglbm.h"
 #include "kernels.hpp"
+#include "state.hpp"
 
 struct cglbm_batch;
 
@@ -23,7 +24,8 @@ struct cglbm_domain {
     double *d_result = nullptr;
     int64_t steps = 0;
     cglbm_layout layout{};
-    cglbm::MarchShape shape{0, 0, 0};  // ty == 0: tuned record or default
+    cglbm::MarchShape shape{0, 0, 0};  // ty == 0: tuned record or default; L == 0: planned chunks
+    cglbm::MarchPlanState plan;         // balanced chunk plan (state.hpp)
     cglbm_batch *batch = nullptr;       // owning batch, if any
     hipStream_t batch_stream = nullptr; // its stream
 };

@@ -33,6 +33,13 @@ struct MarchArgs {
     uint32_t seed;
     int64_t step;
     int store;  // kStoreF32 or kStoreF16 (storage.hpp); spans and strides refer to that layout
+    // Optional work plan (single-domain launches): items[i] = {tile x, tile y, z0, z1}, one per
+    // work-group (z0 == z1: idle). Null: uniform chunks of the launch shape's length.
+    const int4 *items;
+    int nitems;
+    // Optional interface map written by the step: wmap[((tile y * ntx + tile x) * nz + z) * (TY / 2) + w]
+    // = 1 when own wave w of that column tile took the interface form at source plane z, else 0.
+    uint8_t *wmap;
 };
 // Batched launch: descriptors of member domains (one kernel class: q, ybc, forced), a list of
 // work items {domain, tile x, tile y, chunk}, and the device step base.
@@ -78,6 +85,14 @@ hipError_t launch_march_q27_f16(const MarchArgs &a, MarchShape m, hipStream_t s)
 hipError_t launch_march_q19_f32big(const MarchArgs &a, MarchShape m, hipStream_t s);
 hipError_t launch_march_q27_f32big(const MarchArgs &a, MarchShape m, hipStream_t s);
 hipError_t launch_march(const StepArgs &a, int q, hipStream_t s);
+
+// Balanced chunk plan of one domain (plan.hip): from the interface map of a recorded step
+// (ntx * nty column tiles, nz planes, nw own waves per tile), writes nitems work items
+// {tile x, tile y, z0, z1} covering every (column tile, plane) once; unused items are idle.
+constexpr int64_t kPlanMaxShared = 60 * 1024;  // dynamic LDS of the planner: 4 B per column + 1 B per plane
+bool plan_fits(int ntx, int nty, int nz, int nitems);
+hipError_t launch_plan(const uint8_t *wmap, int ntx, int nty, int nz, int nw, float alpha, float e, int4 *items,
+                       int nitems, hipStream_t s);
 
 // FP16-deviation states (storage.hpp, field stride fs) <-> FP32 compact fields [f0, f1), field f of
 // the FP32 side at (f - f0) * xs floats; n cells.

@@ -261,12 +261,12 @@ __device__ __forceinline__ void boundary_planes(const MarchArgs &a, const FieldB
 
 enum : int { kRoleOwn = 0, kRoleRows = 1, kRoleCols = 2, kRoleNone = 3 };
 
-// One work item: tile (tix, tiy) of chunk `chunk` (of nchunk) of the domain described by `a`,
-// executed by the whole work-group. Ends with the work-group synchronised so the caller may run
-// another item with the same LDS.
+// One work item: tile (tix, tiy), target planes z0 .. z1-1 (z0 < z1), of the domain described by
+// `a`, executed by the whole work-group. Ends with the work-group synchronised so the caller may
+// run another item with the same LDS.
 template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST>
-__device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBUF> &sh, int tix, int tiy, int chunk,
-                                           int nchunk) {
+__device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBUF> &sh, int tix, int tiy, int z0,
+                                           int z1) {
     constexpr int NOWN = TX * TY;
     constexpr int RW = ring_waves<TY>();
     constexpr int NRING = 64 * RW;
@@ -276,8 +276,6 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     constexpr uint32_t MX = pair_mask_where<Q>(0), MY = pair_mask_where<Q>(1), MZ = pair_mask_where<Q>(2);
 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int z0 = static_cast<int>((static_cast<int64_t>(chunk) * nz) / nchunk);
-    const int z1 = static_cast<int>((static_cast<int64_t>(chunk + 1) * nz) / nchunk);
     const int Lc = z1 - z0;
     const int x0 = tix * TX, r0 = tiy * TY;
     const int t = threadIdx.x;
@@ -436,6 +434,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
         }
 
         // ---- sources ----------------------------------------------------------------------
+        bool ion = false;  // this lane's source took the interface form
         if (srow >= 0) {
             constexpr int MG = margin<Q, TY>();
             auto lds_push = [&](auto I, float fr, float fb) {
@@ -474,6 +473,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
                     // recolouring term) instead of both forms under divergent masks. D3Q27
                     // (longer interface form, higher register pressure) branches per lane.
                     const bool iface = Q == 19 ? (__builtin_amdgcn_ballot_w64(f.on) != 0) : f.on;
+                    ion = f.on;
                     if (iface) {
                         colour_gradient<Q>(psi_reader(p), f.Fx, f.Fy, f.Fz);
                         iface_finish<Q>(f, cur, c, a.conservative);
@@ -510,6 +510,14 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
                     if (edge) run(std::integral_constant<uint32_t, mask_half(MX & MZ, 1)>{}, std::false_type{}, lds_push);
                     else run(std::integral_constant<uint32_t, mask_half(MX, 1)>{}, std::false_type{}, lds_push);
                 }
+            }
+        }
+        if (a.wmap != nullptr && role == kRoleOwn && !edge) {  // wave-uniform condition
+            const bool any = __builtin_amdgcn_ballot_w64(ion) != 0;
+            if ((t & 63) == 0) {
+                const int ntx = (nx + TX - 1) / TX;
+                const size_t col = static_cast<size_t>(tiy) * ntx + tix;
+                a.wmap[(col * nz + p) * (TY / 2) + t / 64] = any ? 1 : 0;
             }
         }
         __syncthreads();
@@ -556,10 +564,22 @@ __global__ void __launch_bounds__(block_threads<TY>()) CGLBM_MARCH_OCC
     const int per = gridDim.x / 8;
     const int logical = (b % 8) * per + b / 8;  // XCD-contiguous tile ranges
     if (logical >= nblocks) return;
-    const int tix = logical % ntx;
-    const int tiy = (logical / ntx) % nty;
-    const int chunk = logical / (ntx * nty);
-    march_tile<Q, YBC, TY, NBUF, FORCED, ST>(a, sh, tix, tiy, chunk, nchunk);
+    int tix, tiy, z0, z1;
+    if (a.items != nullptr) {
+        const int4 it = a.items[logical];
+        tix = it.x;
+        tiy = it.y;
+        z0 = it.z;
+        z1 = it.w;
+        if (z0 >= z1) return;  // idle item
+    } else {
+        tix = logical % ntx;
+        tiy = (logical / ntx) % nty;
+        const int chunk = logical / (ntx * nty);
+        z0 = static_cast<int>((static_cast<int64_t>(chunk) * a.nz) / nchunk);
+        z1 = static_cast<int>((static_cast<int64_t>(chunk + 1) * a.nz) / nchunk);
+    }
+    march_tile<Q, YBC, TY, NBUF, FORCED, ST>(a, sh, tix, tiy, z0, z1);
 }
 
 // Batched form: many domains of one kernel class in one launch. Each work item names a domain
@@ -585,7 +605,11 @@ __global__ void __launch_bounds__(block_threads<TY>()) CGLBM_MARCH_OCC
     a.A0 = parity ? d.buf1 : d.buf0;
     a.B0 = parity ? d.buf0 : d.buf1;
     a.step = steps_done + 1;
-    march_tile<Q, YBC, TY, NBUF, FORCED, ST>(a, sh, it.y, it.z, it.w, d.nchunk);
+    a.items = nullptr;
+    a.wmap = nullptr;
+    const int z0 = static_cast<int>((static_cast<int64_t>(it.w) * a.nz) / d.nchunk);
+    const int z1 = static_cast<int>((static_cast<int64_t>(it.w + 1) * a.nz) / d.nchunk);
+    march_tile<Q, YBC, TY, NBUF, FORCED, ST>(a, sh, it.y, it.z, z0, z1);
 }
 
 template <int Q, int YBC, int TY, int NBUF, int ST>
@@ -593,7 +617,7 @@ hipError_t launch_shape(const MarchArgs &a, int L, hipStream_t s) {
     const int ntx = (a.nx + TX - 1) / TX;
     const int nty = (a.ny + TY - 1) / TY;
     const int nchunk = (a.nz + L - 1) / L;
-    const int nblocks = ntx * nty * nchunk;
+    const int nblocks = a.items != nullptr ? a.nitems : ntx * nty * nchunk;
     const int grid = (nblocks + 7) / 8 * 8;
     // The body-force variant exists for bounded y (the RTI walls); a periodic forced domain takes
     // the gather path (march_supports).

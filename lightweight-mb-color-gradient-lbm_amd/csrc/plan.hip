@@ -1,0 +1,224 @@
+// Balanced work plan for the z-marching kernel of one domain.
+//
+// A step's cost is not uniform over the domain: a wave whose sources include an interface cell
+// runs the longer interface form (colour gradient, Omega2, recolouring), about 1.45x (D3Q19) to
+// 1.6x (D3Q27) the bulk form. With uniform z-chunks, a work-group whose chunk lies along the
+// interface takes that much longer than one in the bulk, and at sizes where every CU holds one
+// work-group (128^3: 256 work-groups) the step waits for the slowest. The plan cuts each column
+// tile's z range into chunks of about equal modelled cost instead, from the per-wave interface
+// map the kernel writes (MarchArgs::wmap), so that all work-groups of a round finish together.
+// The partition only moves work between work-groups: every cell is updated by the same
+// arithmetic whatever the chunk bounds, so results are bitwise unchanged.
+//
+// Cost model, in units of one bulk plane of a column tile: a plane costs 1 + alpha * m / per,
+// m = the largest number of interface waves on one SIMD (own wave w runs on SIMD w % 4, per =
+// waves per SIMD); a chunk costs its planes plus e (the two edge source planes and the pipeline
+// prologue). The smallest bound T such that cutting every column greedily into chunks of cost
+// <= T needs at most nitems chunks is searched for by trying many bounds at once (one group of
+// threads per bound); greedy cutting gives the fewest chunks for a bound, so T is the best bound
+// for contiguous chunks.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace cglbm {
+namespace {
+
+constexpr int kPlanThreads = 1024;
+constexpr int kMaxCand = 32;   // bounds tried at once (one per group of ncol threads)
+constexpr int kRounds = 3;     // each round narrows the bound interval (ncand + 1)-fold
+
+// Plane levels of one column, four per 32-bit word (column stride a multiple of 4 bytes).
+__device__ __forceinline__ float plane_cost(uint32_t word, int k, float unit) {
+    return 1.0f + unit * static_cast<float>((word >> (8 * k)) & 0xFFu);
+}
+
+// Chunks needed to cut one column greedily so that no chunk costs more than T (the next word of
+// levels is loaded while the current one is cut).
+__device__ int greedy_count(const uint32_t *col, int nz, float unit, float e, float T) {
+    int count = 1;
+    float acc = e;
+    uint32_t next = col[0];
+    for (int z4 = 0; z4 < nz; z4 += 4) {
+        const uint32_t word = next;
+        if (z4 + 4 < nz) next = col[(z4 >> 2) + 1];
+        const int kend = nz - z4 < 4 ? nz - z4 : 4;
+        for (int k = 0; k < kend; ++k) {
+            const float w = plane_cost(word, k, unit);
+            if (acc + w > T) {
+                ++count;
+                acc = e + w;
+            } else {
+                acc += w;
+            }
+        }
+    }
+    return count;
+}
+
+// Level of one (column tile, plane) from its NW interface bytes (0/1 per own wave, wave w on
+// SIMD w % 4): the most interface waves on one SIMD.
+template <int NW>
+__device__ __forceinline__ uint32_t plane_level(const uint8_t *w) {
+    if constexpr (NW == 8) {
+        const uint64_t v = *reinterpret_cast<const uint64_t *>(w);
+        const uint32_t s = static_cast<uint32_t>(v) + static_cast<uint32_t>(v >> 32);  // bytes 0..2, no carry
+        const uint32_t a = (s & 0xFFu) > ((s >> 8) & 0xFFu) ? (s & 0xFFu) : ((s >> 8) & 0xFFu);
+        const uint32_t b = ((s >> 16) & 0xFFu) > (s >> 24) ? ((s >> 16) & 0xFFu) : (s >> 24);
+        return a > b ? a : b;
+    } else if constexpr (NW == 4) {
+        return *reinterpret_cast<const uint32_t *>(w) != 0u ? 1u : 0u;
+    } else {
+        return *reinterpret_cast<const uint16_t *>(w) != 0u ? 1u : 0u;
+    }
+}
+
+template <int NW>
+__global__ void __launch_bounds__(kPlanThreads)
+    plan_kernel(const uint8_t *__restrict__ wmap, int ntx, int nty, int nz, float alpha, float e,
+                int4 *__restrict__ items, int nitems) {
+    extern __shared__ uint32_t smem[];
+    const int ncol = ntx * nty;
+    const int nzw = (nz + 3) / 4;                        // 32-bit words of levels per column
+    const int cs = nzw | 1;                              // column stride, odd: lanes reading one
+                                                         // word of different columns hit distinct banks
+    int *best = reinterpret_cast<int *>(smem);           // ncol chunk counts at the bound s_hi
+    int *cand = best + ncol;                             // ncand x ncol counts of one round
+    uint32_t *lvl = reinterpret_cast<uint32_t *>(cand + kPlanThreads);  // ncol x cs words
+    __shared__ int s_count[kMaxCand];
+    __shared__ float s_lo, s_hi;
+    __shared__ int s_first, s_total;
+    const int tid = threadIdx.x;
+
+    constexpr int per = NW == 8 ? 2 : 1;  // own waves per SIMD
+    const float unit = alpha / static_cast<float>(per);
+    for (int i = tid; i < ncol * nzw; i += blockDim.x) {
+        const int c = i / nzw, zw = i - c * nzw, z4 = zw * 4;
+        uint32_t word = 0u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (z4 + k < nz) word |= plane_level<NW>(wmap + (static_cast<size_t>(c) * nz + z4 + k) * NW) << (8 * k);
+        lvl[c * cs + zw] = word;
+    }
+    __syncthreads();
+
+    // bounds: a chunk of one bulk plane (lo) and a column of interface planes only (hi, widened
+    // against rounding: one chunk per column then fits, feasible as ncol <= nitems)
+    for (int c = tid; c < ncol; c += blockDim.x) best[c] = 1;
+    if (tid == 0) {
+        s_lo = e + 1.0f;
+        s_hi = (e + static_cast<float>(nz) * (1.0f + alpha)) * 1.001f;
+    }
+
+    // search for the smallest feasible bound: ncand bounds per round, spread over (lo, hi)
+    int ncand = static_cast<int>(blockDim.x) / ncol;
+    ncand = ncand < 1 ? 1 : (ncand > kMaxCand ? kMaxCand : ncand);
+    const int j = tid / ncol;
+    for (int round = 0; round < kRounds; ++round) {
+        if (tid < kMaxCand) s_count[tid] = 0;
+        __syncthreads();
+        const float lo = s_lo, hi = s_hi;
+        const float step = (hi - lo) / static_cast<float>(ncand + 1);
+        if (j < ncand) {
+            const float T = lo + step * static_cast<float>(j + 1);
+            int local = 0;
+            if (ncand > 1) {
+                const int c = tid - j * ncol;
+                local = greedy_count(lvl + c * cs, nz, unit, e, T);
+                cand[tid] = local;
+            } else {
+                for (int c = tid; c < ncol; c += blockDim.x) local += greedy_count(lvl + c * cs, nz, unit, e, T);
+            }
+            atomicAdd(&s_count[j], local);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int first = ncand;  // smallest feasible candidate
+            for (int k = ncand - 1; k >= 0; --k)
+                if (s_count[k] <= nitems) first = k;
+            if (first < ncand) s_hi = lo + step * static_cast<float>(first + 1);
+            if (first > 0) s_lo = lo + step * static_cast<float>(first);
+            s_first = first < ncand ? first : -1;
+        }
+        __syncthreads();
+        const int f = s_first;
+        if (f >= 0) {
+            if (ncand > 1) {
+                if (tid < ncol) best[tid] = cand[f * ncol + tid];
+            } else {
+                for (int c = tid; c < ncol; c += blockDim.x)
+                    best[c] = greedy_count(lvl + c * cs, nz, unit, e, s_hi);
+            }
+        }
+        __syncthreads();
+    }
+    const float T = s_hi;
+
+    if (tid == 0) {
+        int off = 0;
+        for (int c = 0; c < ncol; ++c) {
+            const int k = best[c];
+            best[c] = off;
+            off += k;
+        }
+        s_total = off;
+    }
+    __syncthreads();
+    if (s_total > nitems) {
+        // not reached (every bound kept was tested feasible with the same greedy cut); one
+        // chunk per column keeps every cell covered regardless
+        for (int c = tid; c < ncol; c += blockDim.x) items[c] = make_int4(c % ntx, c / ntx, 0, nz);
+        for (int i = ncol + tid; i < nitems; i += blockDim.x) items[i] = make_int4(0, 0, 0, 0);
+        return;
+    }
+    for (int c = tid; c < ncol; c += blockDim.x) {
+        const int tix = c % ntx, tiy = c / ntx;
+        int o = best[c];
+        const int oend = c + 1 < ncol ? best[c + 1] : s_total;
+        int z0 = 0;
+        float acc = e;
+        const uint32_t *col = lvl + c * cs;
+        uint32_t word = 0u;
+        for (int z = 0; z < nz; ++z) {
+            if ((z & 3) == 0) word = col[z >> 2];
+            const float w = plane_cost(word, z & 3, unit);
+            // the cut count equals best[c] (same greedy, same bound); the guard only keeps the
+            // writes inside this column's range, extending its last chunk, should they differ
+            if (acc + w > T && o < oend - 1) {
+                items[o++] = make_int4(tix, tiy, z0, z);
+                z0 = z;
+                acc = e + w;
+            } else {
+                acc += w;
+            }
+        }
+        items[o] = make_int4(tix, tiy, z0, nz);
+    }
+    for (int i = s_total + tid; i < nitems; i += blockDim.x) items[i] = make_int4(0, 0, 0, 0);
+}
+
+}  // namespace
+
+size_t plan_shared(int64_t ncol, int64_t nz) { return static_cast<size_t>(4 * (ncol + kPlanThreads + ncol * (((nz + 3) / 4) | 1))); }
+
+bool plan_fits(int ntx, int nty, int nz, int nitems) {
+    const int64_t ncol = static_cast<int64_t>(ntx) * nty;
+    return ncol >= 1 && ncol <= nitems && static_cast<int64_t>(plan_shared(ncol, nz)) <= kPlanMaxShared;
+}
+
+hipError_t launch_plan(const uint8_t *wmap, int ntx, int nty, int nz, int nw, float alpha, float e, int4 *items,
+                       int nitems, hipStream_t s) {
+    if (!plan_fits(ntx, nty, nz, nitems) || !(e >= 0.0f) || !(alpha >= 0.0f)) return hipErrorInvalidValue;
+    const size_t shm = plan_shared(static_cast<int64_t>(ntx) * nty, nz);
+    switch (nw) {
+    case 8: plan_kernel<8><<<1, kPlanThreads, shm, s>>>(wmap, ntx, nty, nz, alpha, e, items, nitems); break;
+    case 4: plan_kernel<4><<<1, kPlanThreads, shm, s>>>(wmap, ntx, nty, nz, alpha, e, items, nitems); break;
+    case 2: plan_kernel<2><<<1, kPlanThreads, shm, s>>>(wmap, ntx, nty, nz, alpha, e, items, nitems); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace cglbm

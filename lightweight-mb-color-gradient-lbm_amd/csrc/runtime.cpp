@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -154,26 +155,127 @@ int check_moments(const cglbm_moments *m, const char *name) {
 
 namespace {
 std::mutex g_tuned_mu;
-std::map<std::array<int, 5>, MarchShape> g_tuned;
-std::array<int, 5> tuned_key(const cglbm_config &c) { return {c.q, c.ybc, c.nx, c.ny, c.nz}; }
+std::map<std::array<int, 7>, MarchShape> g_tuned;
+// storage and forcing select different kernel instantiations, so they are part of the key
+std::array<int, 7> tuned_key(const cglbm_config &c, int store) {
+    return {c.q, c.ybc, c.nx, c.ny, c.nz, store, c.gravity != 0.0f ? 1 : 0};
+}
 }  // namespace
 
-bool tuned_shape_lookup(const cglbm_config &c, MarchShape *out) {
+bool tuned_shape_lookup(const cglbm_config &c, int store, MarchShape *out) {
     std::lock_guard<std::mutex> lk(g_tuned_mu);
-    auto it = g_tuned.find(tuned_key(c));
+    auto it = g_tuned.find(tuned_key(c, store));
     if (it == g_tuned.end()) return false;
     *out = it->second;
     return true;
 }
 
-void tuned_shape_store(const cglbm_config &c, MarchShape m) {
+void tuned_shape_store(const cglbm_config &c, int store, MarchShape m) {
     std::lock_guard<std::mutex> lk(g_tuned_mu);
-    g_tuned[tuned_key(c)] = m;
+    g_tuned[tuned_key(c, store)] = m;
+}
+
+namespace {
+constexpr int kTX = 32;
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+// Plan cost model per stencil, in bulk planes of a 16-row column tile: the extra cost of a plane
+// whose SIMDs all run interface waves, and the fixed cost of a chunk (prologue and edge planes).
+// A least-squares fit to per-work-group durations of MI355X steps (96^3 and 128^3 drops, chunks
+// of 8 and 16 planes) gives alpha 0.7 / 1.0 and e 3.6 / 3.45 (D3Q19 / D3Q27); planned work-groups
+// run longer than that model says (every CU busy to the end), and a sweep of whole-step times
+// over alpha and e puts the best at the larger alpha kept here.
+// CGLBM_PLAN_ALPHA / CGLBM_PLAN_E override both for tuning sweeps.
+float env_float(const char *name, float dflt) {
+    const char *v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    char *end = nullptr;
+    const float x = std::strtof(v, &end);
+    return (end && *end == '\0' && x >= 0.0f && x <= 100.0f) ? x : dflt;
+}
+float plan_alpha(int q) { return env_float("CGLBM_PLAN_ALPHA", q == 27 ? 1.4f : 1.0f); }
+float plan_overhead(int q) { return env_float("CGLBM_PLAN_E", q == 27 ? 3.45f : 3.6f); }
+
+int plan_items_for(const cglbm_config &c, int ty, int ncu) {
+    const int ncol = ceil_div(c.nx, kTX) * ceil_div(c.ny, ty);
+    return ncu * ceil_div(ncol, ncu);
+}
+
+// Untuned shape of a library-owned domain: 16-row tiles with planned chunks (two exchange
+// buffers where they fit the LDS) when the plan applies and its chunks average at least 4 planes
+// (measured: shorter chunks lose to uniform ones, the fixed cost of a chunk dominating); else
+// the size-aware uniform default.
+MarchShape untuned_shape(const cglbm_config &c, const MarchPlanState *plan, const MarchShape &dflt) {
+    if (!plan || plan->ncu < 1) return dflt;
+    const int ntx = ceil_div(c.nx, kTX), nty = ceil_div(c.ny, 16);
+    const int nitems = plan_items_for(c, 16, plan->ncu);
+    if (nitems > plan->cap || !plan_fits(ntx, nty, c.nz, nitems)) return dflt;
+    if (static_cast<int64_t>(ntx) * nty * c.nz < 4 * static_cast<int64_t>(nitems)) return dflt;
+    return MarchShape{16, 0, c.q == 27 ? 1 : 2};
+}
+
+// The marching launch of one step, with the domain's chunk plan when the shape asks for one
+// (L == 0) and the plan applies; otherwise uniform chunks of fallback_L planes.
+hipError_t launch_march_planned(MarchArgs ma, const cglbm_config &c, MarchShape m, MarchPlanState *plan,
+                                int fallback_L, hipStream_t s) {
+    const bool want = m.L == 0;
+    MarchShape mu = m;
+    if (want) mu.L = fallback_L;  // chunk length of unplanned launches
+    if (!want || !plan || !plan->items || !plan->wmap) return launch_march_args(ma, c.q, mu, s);
+    const int ntx = ceil_div(c.nx, kTX), nty = ceil_div(c.ny, m.ty);
+    const int nitems = plan_items_for(c, m.ty, plan->ncu);
+    if (nitems > plan->cap || !plan_fits(ntx, nty, c.nz, nitems)) return launch_march_args(ma, c.q, mu, s);
+    const bool have = plan->ty == m.ty;
+    const bool record = !have || plan->since + 1 >= plan->period;
+    if (have) {
+        ma.items = plan->items;
+        ma.nitems = nitems;
+    }
+    if (record) ma.wmap = plan->wmap;
+    hipError_t e = launch_march_args(ma, c.q, mu, s);
+    if (e != hipSuccess) return e;
+    if (record) {
+        e = launch_plan(plan->wmap, ntx, nty, c.nz, m.ty / 2, plan_alpha(c.q), plan_overhead(c.q), plan->items, nitems, s);
+        if (e != hipSuccess) {
+            plan->ty = 0;
+            return e;
+        }
+        plan->ty = m.ty;
+        plan->since = 0;
+    } else {
+        ++plan->since;
+    }
+    return hipSuccess;
+}
+}  // namespace
+
+int plan_items(const cglbm_config &c, int ty, int ncu) { return plan_items_for(c, ty, ncu); }
+
+int plan_capacity(const cglbm_config &c, int ncu) {
+    int best = 0;
+    for (int ty : {4, 8, 16}) {
+        const int n = plan_items_for(c, ty, ncu);
+        best = n > best ? n : best;
+    }
+    return best;
+}
+
+int64_t plan_device_bytes(const cglbm_config &c) {
+    // the CU count is not known without a device: bound it by 1024
+    return static_cast<int64_t>(sizeof(int4)) * plan_capacity(c, 1024) + static_cast<int64_t>(plan_wmap_bytes(c));
+}
+
+size_t plan_wmap_bytes(const cglbm_config &c) {
+    size_t best = 0;
+    for (int ty : {4, 8, 16}) {
+        const size_t b = static_cast<size_t>(ceil_div(c.nx, kTX)) * ceil_div(c.ny, ty) * c.nz * (ty / 2);
+        best = b > best ? b : best;
+    }
+    return best;
 }
 
 int launch_step(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
                 const cglbm_layout *L, int64_t step, int kernel, const MarchShape *shape, hipStream_t s, int store,
-                int64_t fs) {
+                int64_t fs, MarchPlanState *plan) {
     StepArgs a = make_step_args(A, B, cfg, L, step);
     if (store == kStoreF16) {
         // Only the z-marching kernel reads and writes the FP16-deviation format.
@@ -182,9 +284,10 @@ int launch_step(const cglbm_moments *A, const cglbm_moments *B, const cglbm_conf
             a.nz < 3 || !march_args_f16(a, A->rhor, B->rhor, fs, ma))
             return fail(CGLBM_EINVAL, "FP16 storage needs the z-marching kernel and a supported case");
         MarchShape m;
+        const MarchShape dflt = march_default_shape(a, cfg->q);
         if (shape && shape->ty > 0) m = *shape;
-        else if (!tuned_shape_lookup(*cfg, &m)) m = march_default_shape(a, cfg->q);
-        const hipError_t e = launch_march_args(ma, cfg->q, m, s);
+        else if (!tuned_shape_lookup(*cfg, store, &m)) m = untuned_shape(*cfg, plan, dflt);
+        const hipError_t e = launch_march_planned(ma, *cfg, m, plan, dflt.L, s);
         if (e != hipSuccess) return hip_fail(e, "launch_march (FP16 storage)");
         return CGLBM_OK;
     }
@@ -194,9 +297,14 @@ int launch_step(const cglbm_moments *A, const cglbm_moments *B, const cglbm_conf
     hipError_t e;
     if (kernel != CGLBM_KERNEL_GATHER && march_ok) {
         MarchShape m;
+        const MarchShape dflt = march_default_shape(a, cfg->q);
         if (shape && shape->ty > 0) m = *shape;
-        else if (!tuned_shape_lookup(*cfg, &m)) m = march_default_shape(a, cfg->q);
-        e = launch_march_shape(a, cfg->q, m, s);
+        else if (!tuned_shape_lookup(*cfg, store, &m)) m = untuned_shape(*cfg, plan, dflt);
+        if ((m.ty != 4 && m.ty != 8 && m.ty != 16) || m.L < 0 || (m.nbuf != 1 && m.nbuf != 2))
+            return fail(CGLBM_EINVAL, "bad marching shape");
+        MarchArgs ma;
+        if (!march_args(a, ma)) return fail(CGLBM_EINVAL, "the z-marching kernel does not support this shape/layout");
+        e = launch_march_planned(ma, *cfg, m, plan, dflt.L, s);
         if (e != hipSuccess) return hip_fail(e, "launch_march");
         return CGLBM_OK;  // boundary planes are written inside the marching step
     }

@@ -19,13 +19,35 @@ BcArgs make_bc_args(const cglbm_moments *B, const cglbm_moments *Aold, const cgl
 // shape == nullptr: the tuned shape recorded for this configuration, else the default.
 // store == kStoreF16: A->rhor / B->rhor are the bases of FP16-deviation states at field stride fs
 // (storage.hpp); the other view pointers are not used.
+// Balanced z-chunk plan of a library-owned domain (plan.hip). A marching shape with L == 0 asks
+// for it: the step after every `period` steps (and the first) records the interface map and is
+// followed by the planner; the steps in between take their chunks from `items`.
+struct MarchPlanState {
+    int4 *items = nullptr;    // cap work items
+    uint8_t *wmap = nullptr;  // interface map of the last recording step (plan_wmap_bytes)
+    int cap = 0;
+    int ncu = 0;              // CUs of the device
+    int ty = 0;               // tile rows the current plan was made for; 0 = none yet
+    int period = 64;
+    int since = 0;            // steps since the plan was made
+};
+// Work items of a planned step for tiles of `ty` rows: one per CU, rounded up to a multiple of
+// the CU count so every column tile gets at least one.
+int plan_items(const cglbm_config &c, int ty, int ncu);
+// Work items to allocate (the largest plan_items over the tile-row choices) and bytes of the
+// interface map for the largest tile-row choice; both together in device bytes.
+int plan_capacity(const cglbm_config &c, int ncu);
+size_t plan_wmap_bytes(const cglbm_config &c);
+int64_t plan_device_bytes(const cglbm_config &c);
+
 int launch_step(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
                 const cglbm_layout *L, int64_t step, int kernel, const MarchShape *shape, hipStream_t s,
-                int store = 0, int64_t fs = 0);
+                int store = 0, int64_t fs = 0, MarchPlanState *plan = nullptr);
 
-// Process-wide record of autotuned z-marching shapes, keyed by (q, ybc, nx, ny, nz).
-bool tuned_shape_lookup(const cglbm_config &c, MarchShape *out);
-void tuned_shape_store(const cglbm_config &c, MarchShape m);
+// Process-wide record of autotuned z-marching shapes, keyed by (q, ybc, nx, ny, nz, storage,
+// body force on/off).
+bool tuned_shape_lookup(const cglbm_config &c, int store, MarchShape *out);
+void tuned_shape_store(const cglbm_config &c, int store, MarchShape m);
 
 inline int stored_ny(const cglbm_config &c) { return c.ybc == CGLBM_Y_PERIODIC ? c.ny : c.ny + 2; }
 inline int64_t stored_cells(const cglbm_config &c) {

@@ -83,6 +83,9 @@ bool march_args(const StepArgs &a, MarchArgs &m) {
     m.seed = a.seed;
     m.step = a.step;
     m.store = big ? kStoreF32Big : kStoreF32;
+    m.items = nullptr;
+    m.nitems = 0;
+    m.wmap = nullptr;
     return true;
 }
 

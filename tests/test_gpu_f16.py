@@ -96,7 +96,7 @@ def test_initialisers_store_rounded_state(oracle):
         np.testing.assert_array_equal(d.get_state(), oracle.quantize_f16(oracle.init_bubble(p, 5.0, 4.0)))
 
 
-@pytest.mark.parametrize("ty,L,nbuf", [(4, 3, 1), (8, 5, 2), (16, 7, 1), (16, 16, 2), (8, 1, 2)])
+@pytest.mark.parametrize("ty,L,nbuf", [(4, 3, 1), (8, 5, 2), (16, 7, 1), (16, 16, 2), (8, 1, 2), (16, 0, 2), (8, 0, 1)])
 def test_tile_shapes(oracle, ty, L, nbuf):
     cg = load_product()
     for kw in (dict(nx=40, ny=20, nz=16, q=19), dict(nx=20, ny=30, nz=14, q=27, ybc=1, jet_radius=4.0)):

@@ -1,0 +1,145 @@
+"""Planned z-chunks of the marching kernel (tile shapes with L = 0).
+
+Every 64 steps a step records which own waves took the interface form, and a one-work-group
+planner cuts each column tile's z range into chunks of about equal modelled cost, one work-group
+per CU. The partition only moves work between work-groups, so results must stay bitwise equal to
+the oracle (small cases) and to uniform chunks (large cases, across re-plans); the plan itself
+must cover every (column tile, plane) exactly once."""
+import numpy as np
+import pytest
+
+from conftest import load_product
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(cg, p):
+    return cg.Config(**{k: getattr(p, k) for k in p.__dataclass_fields__})
+
+
+def _initial(oracle, p):
+    if p.ybc == oracle.Y_JET:
+        return oracle.init_jet(p)
+    if p.ybc == oracle.Y_WALLS:
+        return oracle.init_rti(p, 2.0, 4.0, 8, 0.2, 1)
+    return oracle.init_bubble(p, radius=min(p.nx, p.ny, p.nz) / 5.0, width=4.0)
+
+
+def _check_cover(items, nx, ny, nz, ty):
+    ntx, nty = -(-nx // 32), -(-ny // ty)
+    cover = np.zeros((nty, ntx, nz), dtype=np.int64)
+    for tx, tyi, z0, z1 in items:
+        if z0 == z1:
+            continue
+        assert 0 <= tx < ntx and 0 <= tyi < nty and 0 <= z0 < z1 <= nz, (tx, tyi, z0, z1)
+        cover[tyi, tx, z0:z1] += 1
+    assert (cover == 1).all(), f"planes covered {np.unique(cover)} times"
+    return ntx * nty
+
+
+PLAN_CASES = [
+    # (label, params, tile rows, exchange buffers)
+    ("drop128_q19", dict(nx=128, ny=128, nz=128, q=19, tau=0.8, sigma=0.01, beta=0.7), 16, 2),
+    ("drop96_q27", dict(nx=96, ny=96, nz=96, q=27, tau=0.8, sigma=0.01, beta=0.7), 16, 1),
+    ("jet_q19_ty8", dict(nx=48, ny=64, nz=40, q=19, ybc=1, jet_radius=6.0), 8, 2),
+    ("rti_q27_ty4", dict(nx=40, ny=48, nz=20, q=27, ybc=2, gravity=1e-4, atwood=0.5), 4, 1),
+]
+
+
+@pytest.mark.parametrize("label,kw,ty,nbuf", PLAN_CASES, ids=[c[0] for c in PLAN_CASES])
+def test_plan_covers_every_plane_once(oracle, label, kw, ty, nbuf):
+    cg = load_product()
+    p = oracle.Params(**kw)
+    print(f"plan cover {label}", flush=True)
+    with cg.Domain(_cfg(cg, p)) as dom:
+        dom.set_tile(ty, 0, nbuf)
+        dom.set_state(_initial(oracle, p))
+        assert dom.plan()[1].shape[0] == 0  # a new state has no plan until a step records one
+        dom.step(2)
+        pty, items = dom.plan()
+        assert pty == ty
+        ncol = _check_cover(items, p.nx, p.ny, p.nz, ty)
+        assert items.shape[0] >= ncol and items.shape[0] % 8 == 0
+        busy = items[items[:, 2] < items[:, 3]]
+        assert busy.shape[0] <= items.shape[0]
+
+
+def test_plan_follows_the_interface(oracle):
+    """Chunks through the drop's interface shell are shorter than chunks in the bulk (the
+    greedy cut leaves one short remainder per column, so medians are compared)."""
+    cg = load_product()
+    p = oracle.Params(nx=128, ny=128, nz=128, q=19, tau=0.8, sigma=0.01, beta=0.7)
+    with cg.Domain(_cfg(cg, p)) as dom:
+        dom.set_tile(16, 0, 2)
+        dom.init_bubble(radius=24.0, width=4.0)
+        dom.step(2)
+        _, items = dom.plan()
+        busy = items[items[:, 2] < items[:, 3]]
+        length = busy[:, 3] - busy[:, 2]
+        # column tile (1, 3) crosses the drop (x 32..63, y 48..63); tile (0, 0) lies in the bulk
+        through = length[(busy[:, 0] == 1) & (busy[:, 1] == 3)]
+        bulk = length[(busy[:, 0] == 0) & (busy[:, 1] == 0)]
+        assert np.median(through) < np.median(bulk), (through, bulk)
+        assert through.sum() == bulk.sum() == p.nz
+
+
+PLANNED = [(16, 0, 2), (8, 0, 1), (4, 0, 2)]
+ORACLE_CASES = [
+    ("bubble32_q19", dict(nx=32, ny=32, nz=32, q=19, tau=0.8, sigma=0.01, beta=0.7), 70),
+    ("bubble_q27", dict(nx=40, ny=24, nz=20, q=27, tau=0.8, sigma=0.01, beta=0.7), 12),
+    ("jet_noise_q27", dict(nx=24, ny=40, nz=20, q=27, ybc=1, tau=0.55, sigma=0.005, beta=0.7, jet_radius=5.0,
+                           jet_noise=0.01, seed=9, outlet=1), 12),
+    ("rti_q19", dict(nx=24, ny=48, nz=20, q=19, ybc=2, tau=0.6, sigma=0.002, beta=0.7, gravity=1e-4, atwood=0.5), 12),
+]
+
+
+@pytest.mark.parametrize("shape", PLANNED, ids=[f"ty{a}nb{c}" for a, _, c in PLANNED])
+@pytest.mark.parametrize("label,kw,steps", ORACLE_CASES, ids=[c[0] for c in ORACLE_CASES])
+def test_planned_matches_oracle(oracle, label, kw, steps, shape):
+    """Bitwise equal to the oracle; the 70-step case crosses a re-plan."""
+    cg = load_product()
+    p = oracle.Params(**kw)
+    A = _initial(oracle, p)
+    print(f"planned {label} shape {shape} steps {steps}", flush=True)
+    want = oracle.run(p, A, steps)
+    with cg.Domain(_cfg(cg, p), kernel=cg.KERNEL_MARCH) as dom:
+        dom.set_tile(*shape)
+        dom.set_state(A)
+        dom.step(steps)
+        got = dom.get_state()
+        assert dom.plan()[0] == shape[0]
+    if not np.array_equal(got, want):
+        diff = np.abs(got.astype(np.float64) - want).reshape(11, -1)
+        pytest.fail(f"{label}: cells differing per field {(diff > 0).sum(axis=1)}, max {diff.max(axis=1)}")
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_planned_equals_uniform_at_128(q):
+    """The 128^3 drop over 140 steps (two re-plans): planned chunks (the untuned default) give
+    the same bits as uniform 16-plane chunks, in FP32 and in FP16-deviation storage."""
+    cg = load_product()
+    cfg = cg.Config(nx=128, ny=128, nz=128, q=q, tau=0.8, sigma=0.01, beta=0.7)
+    for storage in (cg.STORE_F32, cg.STORE_F16):
+        states = []
+        for tile in (None, (16, 16, 1)):
+            print(f"q{q} storage {storage} tile {tile}", flush=True)
+            with cg.Domain(cfg, storage=storage) as dom:
+                if tile:
+                    dom.set_tile(*tile)
+                dom.init_bubble(radius=24.0, width=4.0)
+                dom.step(140)
+                states.append(dom.get_state())
+                pty, items = dom.plan()
+                if tile is None:
+                    assert pty == 16 and items.shape[0] > 0  # the default plans at this size
+        assert np.array_equal(states[0], states[1])
+
+
+def test_small_domains_default_to_uniform_chunks():
+    """At 64^3 planned chunks would average 2 planes; the default keeps uniform chunks there."""
+    cg = load_product()
+    cfg = cg.Config(nx=64, ny=64, nz=64, q=19)
+    with cg.Domain(cfg) as dom:
+        dom.init_bubble(radius=12.0, width=4.0)
+        dom.step(3)
+        assert dom.plan()[1].shape[0] == 0
