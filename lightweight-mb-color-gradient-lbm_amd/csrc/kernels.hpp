@@ -89,7 +89,9 @@ hipError_t launch_march(const StepArgs &a, int q, hipStream_t s);
 // Balanced chunk plan of one domain (plan.hip): from the interface map of a recorded step
 // (ntx * nty column tiles, nz planes, nw own waves per tile), writes nitems work items
 // {tile x, tile y, z0, z1} covering every (column tile, plane) once; unused items are idle.
-constexpr int64_t kPlanMaxShared = 60 * 1024;  // dynamic LDS of the planner: 4 B per column + 1 B per plane
+// dynamic LDS of the planner: 20 KB of item staging and sort keys, 4 B per column and per
+// thread, and one byte per (column tile, plane)
+constexpr int64_t kPlanMaxShared = 60 * 1024;
 bool plan_fits(int ntx, int nty, int nz, int nitems);
 hipError_t launch_plan(const uint8_t *wmap, int ntx, int nty, int nz, int nw, float alpha, float e, int4 *items,
                        int nitems, hipStream_t s);

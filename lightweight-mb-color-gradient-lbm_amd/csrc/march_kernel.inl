@@ -263,8 +263,9 @@ enum : int { kRoleOwn = 0, kRoleRows = 1, kRoleCols = 2, kRoleNone = 3 };
 
 // One work item: tile (tix, tiy), target planes z0 .. z1-1 (z0 < z1), of the domain described by
 // `a`, executed by the whole work-group. Ends with the work-group synchronised so the caller may
-// run another item with the same LDS.
-template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST>
+// run another item with the same LDS. REC: also write the interface map a.wmap (a separate
+// instantiation, so steps that do not record carry none of its code).
+template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST, bool REC = false>
 __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBUF> &sh, int tix, int tiy, int z0,
                                            int z1) {
     constexpr int NOWN = TX * TY;
@@ -434,7 +435,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
         }
 
         // ---- sources ----------------------------------------------------------------------
-        bool ion = false;  // this lane's source took the interface form
+        [[maybe_unused]] bool ion = false;  // this lane's source took the interface form (REC)
         if (srow >= 0) {
             constexpr int MG = margin<Q, TY>();
             auto lds_push = [&](auto I, float fr, float fb) {
@@ -473,7 +474,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
                     // recolouring term) instead of both forms under divergent masks. D3Q27
                     // (longer interface form, higher register pressure) branches per lane.
                     const bool iface = Q == 19 ? (__builtin_amdgcn_ballot_w64(f.on) != 0) : f.on;
-                    ion = f.on;
+                    if constexpr (REC) ion = f.on;
                     if (iface) {
                         colour_gradient<Q>(psi_reader(p), f.Fx, f.Fy, f.Fz);
                         iface_finish<Q>(f, cur, c, a.conservative);
@@ -512,12 +513,14 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
                 }
             }
         }
-        if (a.wmap != nullptr && role == kRoleOwn && !edge) {  // wave-uniform condition
-            const bool any = __builtin_amdgcn_ballot_w64(ion) != 0;
-            if ((t & 63) == 0) {
-                const int ntx = (nx + TX - 1) / TX;
-                const size_t col = static_cast<size_t>(tiy) * ntx + tix;
-                a.wmap[(col * nz + p) * (TY / 2) + t / 64] = any ? 1 : 0;
+        if constexpr (REC) {
+            if (role == kRoleOwn && !edge) {  // wave-uniform condition
+                const bool any = __builtin_amdgcn_ballot_w64(ion) != 0;
+                if ((t & 63) == 0) {
+                    const int ntx = (nx + TX - 1) / TX;
+                    const size_t col = static_cast<size_t>(tiy) * ntx + tix;
+                    a.wmap[(col * nz + p) * (TY / 2) + t / 64] = any ? 1 : 0;
+                }
             }
         }
         __syncthreads();
@@ -556,7 +559,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     if constexpr (NBUF == 2) __syncthreads();
 }
 
-template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST>
+template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST, bool REC>
 __global__ void __launch_bounds__(block_threads<TY>()) CGLBM_MARCH_OCC
     march_kernel(MarchArgs a, int ntx, int nty, int nchunk, int nblocks) {
     __shared__ Shared<Q, TY, NBUF> sh;
@@ -579,7 +582,7 @@ __global__ void __launch_bounds__(block_threads<TY>()) CGLBM_MARCH_OCC
         z0 = static_cast<int>((static_cast<int64_t>(chunk) * a.nz) / nchunk);
         z1 = static_cast<int>((static_cast<int64_t>(chunk + 1) * a.nz) / nchunk);
     }
-    march_tile<Q, YBC, TY, NBUF, FORCED, ST>(a, sh, tix, tiy, z0, z1);
+    march_tile<Q, YBC, TY, NBUF, FORCED, ST, REC>(a, sh, tix, tiy, z0, z1);
 }
 
 // Batched form: many domains of one kernel class in one launch. Each work item names a domain
@@ -606,7 +609,6 @@ __global__ void __launch_bounds__(block_threads<TY>()) CGLBM_MARCH_OCC
     a.B0 = parity ? d.buf0 : d.buf1;
     a.step = steps_done + 1;
     a.items = nullptr;
-    a.wmap = nullptr;
     const int z0 = static_cast<int>((static_cast<int64_t>(it.w) * a.nz) / d.nchunk);
     const int z1 = static_cast<int>((static_cast<int64_t>(it.w + 1) * a.nz) / d.nchunk);
     march_tile<Q, YBC, TY, NBUF, FORCED, ST>(a, sh, it.y, it.z, z0, z1);
@@ -623,11 +625,19 @@ hipError_t launch_shape(const MarchArgs &a, int L, hipStream_t s) {
     // the gather path (march_supports).
     if constexpr (YBC != kYPeriodic) {
         if (a.forced) {
-            march_kernel<Q, YBC, TY, NBUF, true, ST><<<grid, block_threads<TY>(), 0, s>>>(a, ntx, nty, nchunk, nblocks);
+            if (a.wmap)
+                march_kernel<Q, YBC, TY, NBUF, true, ST, true>
+                    <<<grid, block_threads<TY>(), 0, s>>>(a, ntx, nty, nchunk, nblocks);
+            else
+                march_kernel<Q, YBC, TY, NBUF, true, ST, false>
+                    <<<grid, block_threads<TY>(), 0, s>>>(a, ntx, nty, nchunk, nblocks);
             return hipGetLastError();
         }
     }
-    march_kernel<Q, YBC, TY, NBUF, false, ST><<<grid, block_threads<TY>(), 0, s>>>(a, ntx, nty, nchunk, nblocks);
+    if (a.wmap)
+        march_kernel<Q, YBC, TY, NBUF, false, ST, true><<<grid, block_threads<TY>(), 0, s>>>(a, ntx, nty, nchunk, nblocks);
+    else
+        march_kernel<Q, YBC, TY, NBUF, false, ST, false><<<grid, block_threads<TY>(), 0, s>>>(a, ntx, nty, nchunk, nblocks);
     return hipGetLastError();
 }
 
