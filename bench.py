@@ -6,7 +6,10 @@ storage, inputs resident in HBM. A step is one fused update of all 2,097,152 cel
 128^3 figure of the same workload is reported alongside.
 
 value = MLUPS = cells * steps / seconds / 1e6 over exactly --steps timed steps (HIP events on
-the domain's stream, bracketed by device synchronisation). roofline.achieved = 88 B per cell
+the domain's stream, bracketed by device synchronisation). Tiles are autotuned on the live state
+and reported as [tile rows, chunk planes, exchange buffers]; chunk planes 0 = planned chunks (the
+library re-cuts the z-chunks every 64 steps from where the interface is, the planner's launches
+included in the timed steps). roofline.achieved = 88 B per cell
 update (11 FP32 fields read + 11 written) * cells / average step duration. cpu_baseline = the
 reference's own update (compiled for the host from the reference sources, oracle/_ref) on a
 bounded sample of the same workload, on this host's cores.

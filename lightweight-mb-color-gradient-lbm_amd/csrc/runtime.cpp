@@ -208,6 +208,8 @@ int plan_items_for(const cglbm_config &c, int ty, int ncu) {
 // faster.
 MarchShape untuned_shape(const cglbm_config &c, const MarchPlanState *plan, const MarchShape &dflt) {
     if (!plan || plan->ncu < 1) return dflt;
+    // a shape forced through the environment (march_default_shape) wins over planning
+    if (std::getenv("CGLBM_MARCH_TY") || std::getenv("CGLBM_MARCH_L") || std::getenv("CGLBM_MARCH_NBUF")) return dflt;
     const int ntx = ceil_div(c.nx, kTX), nty = ceil_div(c.ny, 16);
     const int nitems = plan_items_for(c, 16, plan->ncu);
     if (nitems > plan->cap || !plan_fits(ntx, nty, c.nz, nitems)) return dflt;
