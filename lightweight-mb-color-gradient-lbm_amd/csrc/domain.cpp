@@ -74,6 +74,7 @@ void release(cglbm_domain *d) {
     if (d->d_result) (void)hipFree(d->d_result);
     if (d->plan.items) (void)hipFree(d->plan.items);
     if (d->plan.wmap) (void)hipFree(d->plan.wmap);
+    if (d->plan.itime) (void)hipFree(d->plan.itime);
     if (d->ev0) (void)hipEventDestroy(d->ev0);
     if (d->ev1) (void)hipEventDestroy(d->ev1);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -172,6 +173,7 @@ int cglbm_domain_create(const cglbm_config *cfg, int32_t storage, cglbm_domain *
         e = hipMalloc(&d->plan.items, sizeof(int4) * d->plan.cap);
     }
     if (e == hipSuccess) e = hipMalloc(&d->plan.wmap, plan_wmap_bytes(*cfg));
+    if (e == hipSuccess) e = hipMalloc(&d->plan.itime, sizeof(uint32_t) * d->plan.cap);
     if (e == hipSuccess) e = hipMemsetAsync(d->buf[0], 0, bytes, d->stream);
     if (e == hipSuccess) e = hipMemsetAsync(d->buf[1], 0, bytes, d->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);

@@ -40,6 +40,9 @@ struct MarchArgs {
     // Optional interface map written by the step: wmap[((tile y * ntx + tile x) * nz + z) * (TY / 2) + w]
     // = 1 when own wave w of that column tile took the interface form at source plane z, else 0.
     uint8_t *wmap;
+    // Optional, with items and wmap: itime[i] = duration of work item i in wall-clock ticks
+    // (100 MHz), written by the recording step for the planner's measured correction.
+    uint32_t *itime;
 };
 // Batched launch: descriptors of member domains (one kernel class: q, ybc, forced), a list of
 // work items {domain, tile x, tile y, chunk}, and the device step base.
@@ -93,8 +96,10 @@ hipError_t launch_march(const StepArgs &a, int q, hipStream_t s);
 // thread, and one byte per (column tile, plane)
 constexpr int64_t kPlanMaxShared = 60 * 1024;
 bool plan_fits(int ntx, int nty, int nz, int nitems);
-hipError_t launch_plan(const uint8_t *wmap, int ntx, int nty, int nz, int nw, float alpha, float e, int4 *items,
-                       int nitems, hipStream_t s);
+// itime (may be null): durations of the items currently in `items`, timed by the recording step;
+// the planner then scales its modelled plane costs by each item's measured over modelled cost.
+hipError_t launch_plan(const uint8_t *wmap, const uint32_t *itime, int ntx, int nty, int nz, int nw, float alpha,
+                       float e, int4 *items, int nitems, hipStream_t s);
 
 // FP16-deviation states (storage.hpp, field stride fs) <-> FP32 compact fields [f0, f1), field f of
 // the FP32 side at (f - f0) * xs floats; n cells.

@@ -355,27 +355,29 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
         psi_r[k] = 0.0f;
         psi_b[k] = 1.0f;
     }
-    auto psi_load = [&](int z) {
+    auto psi_load_into = [&](int z, float (&r)[NPASS], float (&b)[NPASS]) {
         const int zz = wrapz(z);
 #pragma unroll
         for (int k = 0; k < NPASS; ++k) {
             if (psi_row[k] >= 0) {
                 const uint32_t id = cell(psi_x[k], psi_row[k], zz);
-                psi_r[k] = Abuf.ld(0, id);
-                psi_b[k] = Abuf.ld(1, id);
+                r[k] = Abuf.ld(0, id);
+                b[k] = Abuf.ld(1, id);
             }
         }
     };
-    auto psi_store = [&](int slot) {
+    auto psi_store_from = [&](int slot, const float (&r)[NPASS], const float (&b)[NPASS]) {
 #pragma unroll
         for (int k = 0; k < NPASS; ++k) {
             if (psi_row[k] > -2) {
                 const int ci = (t - NOWN) + k * NRING;
-                const float v = psi_row[k] >= 0 ? (psi_r[k] - psi_b[k]) / (psi_r[k] + psi_b[k]) : 0.0f;
+                const float v = psi_row[k] >= 0 ? (r[k] - b[k]) / (r[k] + b[k]) : 0.0f;
                 sh.psi[slot][ci / PX][ci % PX] = v;
             }
         }
     };
+    auto psi_load = [&](int z) { psi_load_into(z, psi_r, psi_b); };
+    auto psi_store = [&](int slot) { psi_store_from(slot, psi_r, psi_b); };
     auto psi_slot = [&](int z) { return (z - (z0 - 2)) % kPsiSlots; };
 
     // psi of neighbour c_j of this lane's source (forces.cuh:43-50; bounded y limited as
@@ -402,10 +404,19 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
 
     // ---- prologue: psi planes z0-2 .. z0+1 (ring lanes), first moments ---------------------
     if (ring) {
+        // all four planes' loads in flight at once: one memory latency for the prologue
+        float pr[4][NPASS], pb[4][NPASS];
+#pragma unroll
         for (int q = 0; q < 4; ++q) {
-            psi_load(z0 - 2 + q);
-            psi_store(q);
+#pragma unroll
+            for (int k = 0; k < NPASS; ++k) {
+                pr[q][k] = 0.0f;
+                pb[q][k] = 1.0f;
+            }
+            psi_load_into(z0 - 2 + q, pr[q], pb[q]);
         }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) psi_store_from(q, pr[q], pb[q]);
     }
     Mom nxt{};
 #if CGLBM_MARCH_PREFETCH
@@ -419,6 +430,13 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
         const int pz = wrapz(p);
         const bool edge = (k == 0) || (k == kend);
         const int xb = NBUF == 2 ? (k & 1) : 0;
+        // ring lanes: psi of plane p + 3 (first read at step k + 2). The 16-row D3Q19 tile loads
+        // it here and stores it after the sources, hiding the load behind the source work (1-1.5%
+        // measured); elsewhere the registers held across the sources cost more than that (D3Q27
+        // and 4-row tiles 2-5% slower), and the load stays next to its store.
+        constexpr bool kEarlyPsi = Q == 19 && TY == 16;
+        const bool psi_next = ring && k + 3 <= kend + 1;
+        if (kEarlyPsi && psi_next) psi_load(p + 3);
         Held<Q> &HA = H0;  // target p - 1
         Held<Q> &HB = H1;  // target p
         Held<Q> &HC = H2;  // target p + 1
@@ -525,9 +543,9 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
         }
         __syncthreads();
 
-        // ---- ring lanes: psi two planes ahead (plane p + 3, first read at step k + 2) ------
-        if (ring && k + 3 <= kend + 1) {
-            psi_load(p + 3);
+        // ---- ring lanes: psi two planes ahead (plane p + 3) -----------------------------------
+        if (psi_next) {
+            if (!kEarlyPsi) psi_load(p + 3);
             psi_store(psi_slot(p + 3));
         }
 
@@ -563,6 +581,8 @@ template <int Q, int YBC, int TY, int NBUF, bool FORCED, int ST, bool REC>
 __global__ void __launch_bounds__(block_threads<TY>()) CGLBM_MARCH_OCC
     march_kernel(MarchArgs a, int ntx, int nty, int nchunk, int nblocks) {
     __shared__ Shared<Q, TY, NBUF> sh;
+    [[maybe_unused]] uint64_t t_start = 0;
+    if constexpr (REC) t_start = wall_clock64();
     const int b = blockIdx.x;
     const int per = gridDim.x / 8;
     const int logical = (b % 8) * per + b / 8;  // XCD-contiguous tile ranges
@@ -583,6 +603,10 @@ __global__ void __launch_bounds__(block_threads<TY>()) CGLBM_MARCH_OCC
         z1 = static_cast<int>((static_cast<int64_t>(chunk + 1) * a.nz) / nchunk);
     }
     march_tile<Q, YBC, TY, NBUF, FORCED, ST, REC>(a, sh, tix, tiy, z0, z1);
+    if constexpr (REC) {
+        // march_tile ends with the work-group synchronised: every wave is done with the item
+        if (a.itime != nullptr && threadIdx.x == 0) a.itime[logical] = static_cast<uint32_t>(wall_clock64() - t_start);
+    }
 }
 
 // Batched form: many domains of one kernel class in one launch. Each work item names a domain
@@ -609,6 +633,7 @@ __global__ void __launch_bounds__(block_threads<TY>()) CGLBM_MARCH_OCC
     a.B0 = parity ? d.buf0 : d.buf1;
     a.step = steps_done + 1;
     a.items = nullptr;
+    a.itime = nullptr;
     const int z0 = static_cast<int>((static_cast<int64_t>(it.w) * a.nz) / d.nchunk);
     const int z1 = static_cast<int>((static_cast<int64_t>(it.w + 1) * a.nz) / d.nchunk);
     march_tile<Q, YBC, TY, NBUF, FORCED, ST>(a, sh, it.y, it.z, z0, z1);

@@ -13,7 +13,10 @@
 // Cost model, in units of one bulk plane of a column tile: a plane costs 1 + alpha * m / per,
 // m = the largest number of interface waves on one SIMD (own wave w runs on SIMD w % 4, per =
 // waves per SIMD); a chunk costs its planes plus e (the two edge source planes and the pipeline
-// prologue). The smallest bound T such that cutting every column greedily into chunks of cost
+// prologue). When the recording step ran on a plan, it also timed that plan's work items, and
+// each item's planes are scaled by its measured over modelled cost (relative to all items), so
+// what the model misses at a place is corrected at the next cut; the first plan of a state is
+// re-made one step later for that. The smallest bound T such that cutting every column greedily into chunks of cost
 // <= T needs at most nitems chunks is searched for by trying many bounds at once (one group of
 // threads per bound); greedy cutting gives the fewest chunks for a bound, so T is the best bound
 // for contiguous chunks.
@@ -33,14 +36,19 @@ constexpr int kMaxCand = 32;   // bounds tried at once (one per group of ncol th
 constexpr int kMaxRounds = 16;
 constexpr float kTolerance = 0.1f;
 
-// Plane levels of one column, four per 32-bit word (column stride a multiple of 4 bytes).
-__device__ __forceinline__ float plane_cost(uint32_t word, int k, float unit) {
-    return 1.0f + unit * static_cast<float>((word >> (8 * k)) & 0xFFu);
+// Plane costs of one column in 1/32 of a bulk plane, one byte each, four per 32-bit word.
+constexpr float kCostUnit = 1.0f / 32.0f;
+__device__ __forceinline__ float plane_cost(uint32_t word, int k) {
+    return kCostUnit * static_cast<float>((word >> (8 * k)) & 0xFFu);
+}
+__device__ __forceinline__ uint32_t cost_byte(float w) {
+    const float q = rintf(w * 32.0f);
+    return q < 1.0f ? 1u : (q > 255.0f ? 255u : static_cast<uint32_t>(q));
 }
 
 // Chunks needed to cut one column greedily so that no chunk costs more than T (the next word of
-// levels is loaded while the current one is cut).
-__device__ int greedy_count(const uint32_t *col, int nz, float unit, float e, float T) {
+// costs is loaded while the current one is cut).
+__device__ int greedy_count(const uint32_t *col, int nz, float e, float T) {
     int count = 1;
     float acc = e;
     uint32_t next = col[0];
@@ -49,7 +57,7 @@ __device__ int greedy_count(const uint32_t *col, int nz, float unit, float e, fl
         if (z4 + 4 < nz) next = col[(z4 >> 2) + 1];
         const int kend = nz - z4 < 4 ? nz - z4 : 4;
         for (int k = 0; k < kend; ++k) {
-            const float w = plane_cost(word, k, unit);
+            const float w = plane_cost(word, k);
             if (acc + w > T) {
                 ++count;
                 acc = e + w;
@@ -80,8 +88,8 @@ __device__ __forceinline__ uint32_t plane_level(const uint8_t *w) {
 
 template <int NW>
 __global__ void __launch_bounds__(kPlanThreads)
-    plan_kernel(const uint8_t *__restrict__ wmap, int ntx, int nty, int nz, float alpha, float e,
-                int4 *__restrict__ items, int nitems) {
+    plan_kernel(const uint8_t *__restrict__ wmap, const uint32_t *__restrict__ itime, int ntx, int nty, int nz,
+                float alpha, float e, int4 *__restrict__ items, int nitems) {
     extern __shared__ int4 smem[];
     const int ncol = ntx * nty;
     const int nzw = (nz + 3) / 4;                        // 32-bit words of levels per column
@@ -91,30 +99,92 @@ __global__ void __launch_bounds__(kPlanThreads)
     uint32_t *key = reinterpret_cast<uint32_t *>(staged + kPlanThreads);  // kPlanThreads sort keys
     int *best = reinterpret_cast<int *>(key + kPlanThreads);  // ncol chunk counts at the bound s_hi
     int *cand = best + ncol;                             // ncand x ncol counts of one round
-    uint32_t *lvl = reinterpret_cast<uint32_t *>(cand + kPlanThreads);  // ncol x cs words
+    uint32_t *lvl = reinterpret_cast<uint32_t *>(cand + kPlanThreads);  // ncol x cs words of costs
     __shared__ int s_count[kMaxCand];
-    __shared__ float s_lo, s_hi;
+    __shared__ float s_lo, s_hi, s_sum_t, s_sum_m;
     __shared__ int s_first, s_total;
+    __shared__ unsigned int s_maxc;
     const int tid = threadIdx.x;
 
     constexpr int per = NW == 8 ? 2 : 1;  // own waves per SIMD
     const float unit = alpha / static_cast<float>(per);
+    if (tid == 0) {
+        s_maxc = 0u;
+        s_sum_t = 0.0f;
+        s_sum_m = 0.0f;
+    }
     for (int i = tid; i < ncol * nzw; i += blockDim.x) {
         const int c = i / nzw, zw = i - c * nzw, z4 = zw * 4;
         uint32_t word = 0u;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            if (z4 + k < nz) word |= plane_level<NW>(wmap + (static_cast<size_t>(c) * nz + z4 + k) * NW) << (8 * k);
+            if (z4 + k < nz) {
+                const uint32_t lv = plane_level<NW>(wmap + (static_cast<size_t>(c) * nz + z4 + k) * NW);
+                word |= cost_byte(1.0f + unit * static_cast<float>(lv)) << (8 * k);
+            }
         lvl[c * cs + zw] = word;
     }
     __syncthreads();
 
-    // bounds: a chunk of one bulk plane (lo) and a column of interface planes only (hi, widened
+    // Measured correction (itime: durations of the items of the previous plan, still in `items`,
+    // timed by the step that wrote the map): each previous item's planes are scaled by its
+    // duration over its modelled cost, relative to the same ratio over all items, clamped to
+    // [0.5, 2]. The model's errors that belong to a place (the halo work of interface neighbours,
+    // the mix of interface waves within a plane) are so carried into the next cut.
+    float *ratio = reinterpret_cast<float *>(cand);  // per previous item, before the search uses cand
+    if (itime != nullptr) {
+        for (int i = tid; i < nitems; i += blockDim.x) {
+            const int4 it = items[i];
+            float r = 0.0f;
+            if (it.z < it.w && it.x >= 0 && it.x < ntx && it.y >= 0 && it.y < nty && it.z >= 0 && it.w <= nz) {
+                const uint32_t *col = lvl + (it.y * ntx + it.x) * cs;
+                float m = e;
+                for (int z = it.z; z < it.w; ++z) m += plane_cost(col[z >> 2], z & 3);
+                const float t = static_cast<float>(itime[i]);
+                if (t > 0.0f) {
+                    r = t / m;
+                    atomicAdd(&s_sum_t, t);
+                    atomicAdd(&s_sum_m, m);
+                }
+            }
+            ratio[i] = r;
+        }
+        __syncthreads();
+        const float mean = s_sum_m > 0.0f ? s_sum_t / s_sum_m : 0.0f;
+        for (int i = tid; i < nitems; i += blockDim.x) {
+            const int4 it = items[i];
+            if (ratio[i] > 0.0f && mean > 0.0f) {
+                float r = ratio[i] / mean;
+                r = r < 0.5f ? 0.5f : (r > 2.0f ? 2.0f : r);
+                uint32_t *col = lvl + (it.y * ntx + it.x) * cs;
+                // planes of one item are this thread's alone (items partition the domain), but a
+                // word may hold planes of two items: update its bytes with atomics
+                for (int z = it.z; z < it.w; ++z) {
+                    const int sh = 8 * (z & 3);
+                    const uint32_t old = (col[z >> 2] >> sh) & 0xFFu;
+                    const uint32_t nw = cost_byte(static_cast<float>(old) * kCostUnit * r);
+                    atomicXor(&col[z >> 2], (old ^ nw) << sh);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < ncol * nzw; i += blockDim.x) {
+        const uint32_t w = lvl[(i / nzw) * cs + i % nzw];
+        uint32_t mx = w & 0xFFu;
+        mx = ((w >> 8) & 0xFFu) > mx ? ((w >> 8) & 0xFFu) : mx;
+        mx = ((w >> 16) & 0xFFu) > mx ? ((w >> 16) & 0xFFu) : mx;
+        mx = (w >> 24) > mx ? (w >> 24) : mx;
+        atomicMax(&s_maxc, mx);
+    }
+    __syncthreads();
+
+    // bounds: a chunk of one cheapest plane (lo) and a column of costliest planes (hi, widened
     // against rounding: one chunk per column then fits, feasible as ncol <= nitems)
     for (int c = tid; c < ncol; c += blockDim.x) best[c] = 1;
     if (tid == 0) {
-        s_lo = e + 1.0f;
-        s_hi = (e + static_cast<float>(nz) * (1.0f + alpha)) * 1.001f;
+        s_lo = e + kCostUnit;
+        s_hi = (e + static_cast<float>(nz) * kCostUnit * static_cast<float>(s_maxc)) * 1.001f;
     }
 
     // search for the smallest feasible bound: ncand bounds per round, spread over (lo, hi)
@@ -132,10 +202,10 @@ __global__ void __launch_bounds__(kPlanThreads)
             int local = 0;
             if (ncand > 1) {
                 const int c = tid - j * ncol;
-                local = greedy_count(lvl + c * cs, nz, unit, e, T);
+                local = greedy_count(lvl + c * cs, nz, e, T);
                 cand[tid] = local;
             } else {
-                for (int c = tid; c < ncol; c += blockDim.x) local += greedy_count(lvl + c * cs, nz, unit, e, T);
+                for (int c = tid; c < ncol; c += blockDim.x) local += greedy_count(lvl + c * cs, nz, e, T);
             }
             atomicAdd(&s_count[j], local);
         }
@@ -155,7 +225,7 @@ __global__ void __launch_bounds__(kPlanThreads)
                 if (tid < ncol) best[tid] = cand[f * ncol + tid];
             } else {
                 for (int c = tid; c < ncol; c += blockDim.x)
-                    best[c] = greedy_count(lvl + c * cs, nz, unit, e, s_hi);
+                    best[c] = greedy_count(lvl + c * cs, nz, e, s_hi);
             }
         }
         __syncthreads();
@@ -193,7 +263,7 @@ __global__ void __launch_bounds__(kPlanThreads)
         uint32_t word = 0u;
         for (int z = 0; z < nz; ++z) {
             if ((z & 3) == 0) word = col[z >> 2];
-            const float w = plane_cost(word, z & 3, unit);
+            const float w = plane_cost(word, z & 3);
             // the cut count equals best[c] (same greedy, same bound); the guard only keeps the
             // writes inside this column's range, extending its last chunk, should they differ
             if (acc + w > T && o < oend - 1) {
@@ -249,14 +319,14 @@ bool plan_fits(int ntx, int nty, int nz, int nitems) {
            static_cast<int64_t>(plan_shared(ncol, nz)) <= kPlanMaxShared;
 }
 
-hipError_t launch_plan(const uint8_t *wmap, int ntx, int nty, int nz, int nw, float alpha, float e, int4 *items,
-                       int nitems, hipStream_t s) {
+hipError_t launch_plan(const uint8_t *wmap, const uint32_t *itime, int ntx, int nty, int nz, int nw, float alpha,
+                       float e, int4 *items, int nitems, hipStream_t s) {
     if (!plan_fits(ntx, nty, nz, nitems) || !(e >= 0.0f) || !(alpha >= 0.0f)) return hipErrorInvalidValue;
     const size_t shm = plan_shared(static_cast<int64_t>(ntx) * nty, nz);
     switch (nw) {
-    case 8: plan_kernel<8><<<1, kPlanThreads, shm, s>>>(wmap, ntx, nty, nz, alpha, e, items, nitems); break;
-    case 4: plan_kernel<4><<<1, kPlanThreads, shm, s>>>(wmap, ntx, nty, nz, alpha, e, items, nitems); break;
-    case 2: plan_kernel<2><<<1, kPlanThreads, shm, s>>>(wmap, ntx, nty, nz, alpha, e, items, nitems); break;
+    case 8: plan_kernel<8><<<1, kPlanThreads, shm, s>>>(wmap, itime, ntx, nty, nz, alpha, e, items, nitems); break;
+    case 4: plan_kernel<4><<<1, kPlanThreads, shm, s>>>(wmap, itime, ntx, nty, nz, alpha, e, items, nitems); break;
+    case 2: plan_kernel<2><<<1, kPlanThreads, shm, s>>>(wmap, itime, ntx, nty, nz, alpha, e, items, nitems); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -237,16 +237,20 @@ hipError_t launch_march_planned(MarchArgs ma, const cglbm_config &c, MarchShape 
         ma.nitems = nitems;
     }
     if (record) ma.wmap = plan->wmap;
+    const bool timed = record && have && plan->itime;  // the items of the current plan get timed
+    if (timed) ma.itime = plan->itime;
     hipError_t e = launch_march_args(ma, c.q, mu, s);
     if (e != hipSuccess) return e;
     if (record) {
-        e = launch_plan(plan->wmap, ntx, nty, c.nz, m.ty / 2, plan_alpha(c.q), plan_overhead(c.q), plan->items, nitems, s);
+        e = launch_plan(plan->wmap, timed ? plan->itime : nullptr, ntx, nty, c.nz, m.ty / 2, plan_alpha(c.q),
+                        plan_overhead(c.q), plan->items, nitems, s);
         if (e != hipSuccess) {
             plan->ty = 0;
             return e;
         }
+        // the first plan of a state is re-made at the next step, from its own measured durations
+        plan->since = have ? 0 : plan->period - 1;
         plan->ty = m.ty;
-        plan->since = 0;
     } else {
         ++plan->since;
     }
@@ -267,7 +271,8 @@ int plan_capacity(const cglbm_config &c, int ncu) {
 
 int64_t plan_device_bytes(const cglbm_config &c) {
     // the CU count is not known without a device: bound it by 1024
-    return static_cast<int64_t>(sizeof(int4)) * plan_capacity(c, 1024) + static_cast<int64_t>(plan_wmap_bytes(c));
+    return static_cast<int64_t>(sizeof(int4) + sizeof(uint32_t)) * plan_capacity(c, 1024) +
+           static_cast<int64_t>(plan_wmap_bytes(c));
 }
 
 size_t plan_wmap_bytes(const cglbm_config &c) {

@@ -25,6 +25,7 @@ BcArgs make_bc_args(const cglbm_moments *B, const cglbm_moments *Aold, const cgl
 struct MarchPlanState {
     int4 *items = nullptr;    // cap work items
     uint8_t *wmap = nullptr;  // interface map of the last recording step (plan_wmap_bytes)
+    uint32_t *itime = nullptr;  // cap item durations of the last recording step
     int cap = 0;
     int ncu = 0;              // CUs of the device
     int ty = 0;               // tile rows the current plan was made for; 0 = none yet

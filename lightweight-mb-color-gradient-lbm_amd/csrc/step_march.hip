@@ -86,6 +86,7 @@ bool march_args(const StepArgs &a, MarchArgs &m) {
     m.items = nullptr;
     m.nitems = 0;
     m.wmap = nullptr;
+    m.itime = nullptr;
     return true;
 }
 
