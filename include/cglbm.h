@@ -127,8 +127,9 @@ int cglbm_domain_set_kernel(cglbm_domain *dom, int32_t kernel);
  * two barriers). L = 0 asks for planned chunks: every 64 steps the step records which waves took
  * the interface form and a planner cuts each column tile's z range into chunks of equal modelled
  * cost, one work-group per CU (results are bitwise the same for every shape). ty = 0 returns to
- * automatic selection: the autotuned record, else planned 16-row chunks where they pay (cubes of
- * about 96^3 to 192^3), else a size-aware uniform shape. */
+ * automatic selection: the autotuned record, else planned 16-row chunks (when they average at
+ * least 2 planes and the planner's LDS holds a byte per (column tile, plane): cubes from 64^3
+ * to 256^3), else a size-aware uniform shape. */
 int cglbm_domain_set_tile(cglbm_domain *dom, int32_t ty, int32_t L, int32_t nbuf);
 /* Autotuner: times each candidate tile shape (planned chunks, L = 0, among them) for
  * steps_per_candidate updates from the current state (the state does not advance), keeps the

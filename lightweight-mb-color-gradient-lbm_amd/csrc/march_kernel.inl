@@ -405,6 +405,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     // ---- prologue: psi planes z0-2 .. z0+1 (ring lanes), first moments ---------------------
     if (ring) {
         // all four planes' loads in flight at once: one memory latency for the prologue
+        // (measured 3% on the batched step of 256 small domains, 1% on a 128^3 domain)
         float pr[4][NPASS], pb[4][NPASS];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -430,13 +431,6 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
         const int pz = wrapz(p);
         const bool edge = (k == 0) || (k == kend);
         const int xb = NBUF == 2 ? (k & 1) : 0;
-        // ring lanes: psi of plane p + 3 (first read at step k + 2). The 16-row D3Q19 tile loads
-        // it here and stores it after the sources, hiding the load behind the source work (1-1.5%
-        // measured); elsewhere the registers held across the sources cost more than that (D3Q27
-        // and 4-row tiles 2-5% slower), and the load stays next to its store.
-        constexpr bool kEarlyPsi = Q == 19 && TY == 16;
-        const bool psi_next = ring && k + 3 <= kend + 1;
-        if (kEarlyPsi && psi_next) psi_load(p + 3);
         Held<Q> &HA = H0;  // target p - 1
         Held<Q> &HB = H1;  // target p
         Held<Q> &HC = H2;  // target p + 1
@@ -543,9 +537,11 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
         }
         __syncthreads();
 
-        // ---- ring lanes: psi two planes ahead (plane p + 3) -----------------------------------
-        if (psi_next) {
-            if (!kEarlyPsi) psi_load(p + 3);
+        // ---- ring lanes: psi two planes ahead (plane p + 3, first read at step k + 2) ------
+        // (loading it before the sources instead, to hide the load, measured no faster on
+        // single domains and 2.4% slower on batches: the registers it holds across the sources)
+        if (ring && k + 3 <= kend + 1) {
+            psi_load(p + 3);
             psi_store(psi_slot(p + 3));
         }
 

@@ -201,11 +201,10 @@ int plan_items_for(const cglbm_config &c, int ty, int ncu) {
 }
 
 // Untuned shape of a library-owned domain: 16-row tiles with planned chunks (two exchange
-// buffers where they fit the LDS) when the plan applies, its chunks average at least 4 planes and
-// uniform 16-plane chunks would fill the CUs at most four times over; else the size-aware uniform
-// default. Measured on drops: planned chunks win from 96^3 to 192^3; at 64^3 (2-plane chunks)
-// and 256^3 (eight rounds of uniform work-groups balance by themselves) uniform chunks are
-// faster.
+// buffers where they fit the LDS) when the plan fits the planner and its chunks average at least
+// 2 planes; else the size-aware uniform default. Measured on drops from 64^3 to 256^3, planned
+// chunks (with the measured correction) are as fast as the best uniform shape at 64^3 and 4-30%
+// faster from 96^3 up.
 MarchShape untuned_shape(const cglbm_config &c, const MarchPlanState *plan, const MarchShape &dflt) {
     if (!plan || plan->ncu < 1) return dflt;
     // a shape forced through the environment (march_default_shape) wins over planning
@@ -213,9 +212,7 @@ MarchShape untuned_shape(const cglbm_config &c, const MarchPlanState *plan, cons
     const int ntx = ceil_div(c.nx, kTX), nty = ceil_div(c.ny, 16);
     const int nitems = plan_items_for(c, 16, plan->ncu);
     if (nitems > plan->cap || !plan_fits(ntx, nty, c.nz, nitems)) return dflt;
-    const int64_t planes = static_cast<int64_t>(ntx) * nty * c.nz;
-    if (planes < 4 * static_cast<int64_t>(nitems)) return dflt;
-    if (static_cast<int64_t>(ntx) * nty * ceil_div(c.nz, 16) > 4 * static_cast<int64_t>(plan->ncu)) return dflt;
+    if (static_cast<int64_t>(ntx) * nty * c.nz < 2 * static_cast<int64_t>(nitems)) return dflt;
     return MarchShape{16, 0, c.q == 27 ? 1 : 2};
 }
 

@@ -138,10 +138,11 @@ def test_planned_equals_uniform_at_128(q):
         assert np.array_equal(states[0], states[1])
 
 
-@pytest.mark.parametrize("n,planned", [(64, False), (96, True), (192, True), (256, False)])
-def test_default_plans_only_where_it_pays(n, planned):
-    """The untuned default plans from 96^3 to 192^3; at 64^3 planned chunks would average 2
-    planes and at 256^3 uniform chunks fill the CUs eight times over, so both keep uniform ones."""
+@pytest.mark.parametrize("n,planned", [(48, False), (64, True), (128, True), (256, True), (320, False)])
+def test_default_plans_where_it_fits(n, planned):
+    """The untuned default plans its chunks when they average at least 2 planes (not at 48^3) and
+    the planner's LDS holds the domain's column tiles (not at 320^3, 200 column tiles of 320
+    planes); otherwise it keeps the size-aware uniform shape."""
     cg = load_product()
     cfg = cg.Config(nx=n, ny=n, nz=n, q=19)
     with cg.Domain(cfg) as dom:
