@@ -335,6 +335,7 @@ int cglbm_batch_remove(cglbm_batch *b, cglbm_domain *d) {
     b->members.erase(it);
     d->batch = nullptr;
     d->batch_stream = nullptr;
+    d->plan.ty = 0;  // its state moved on in the batch: a planned shape records a fresh map
     b->dirty = true;
     return CGLBM_OK;
 }

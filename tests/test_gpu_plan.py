@@ -43,6 +43,8 @@ PLAN_CASES = [
     ("drop96_q27", dict(nx=96, ny=96, nz=96, q=27, tau=0.8, sigma=0.01, beta=0.7), 16, 1),
     ("jet_q19_ty8", dict(nx=48, ny=64, nz=40, q=19, ybc=1, jet_radius=6.0), 8, 2),
     ("rti_q27_ty4", dict(nx=40, ny=48, nz=20, q=27, ybc=2, gravity=1e-4, atwood=0.5), 4, 1),
+    # more column tiles (17 x 16 = 272) than CUs: two rounds of work-groups, two bounds per round
+    ("wide_q19", dict(nx=512, ny=272, nz=8, q=19), 16, 2),
 ]
 
 
