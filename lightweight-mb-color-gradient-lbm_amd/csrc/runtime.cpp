@@ -195,6 +195,10 @@ float env_float(const char *name, float dflt) {
 float plan_alpha(int q) { return env_float("CGLBM_PLAN_ALPHA", q == 27 ? 1.4f : 1.0f); }
 float plan_overhead(int q) { return env_float("CGLBM_PLAN_E", q == 27 ? 3.45f : 3.6f); }
 
+// Work items of a plan: one work-group per CU, or whole rounds of them when there are more
+// column tiles than CUs. (More rounds with shorter chunks were measured slower: 192^3 and 256^3
+// drops and a 128 x 512 x 128 RTI case lose 6-18% with two or four rounds, each chunk's fixed
+// cost and its halo reads outweighing the finer balance.)
 int plan_items_for(const cglbm_config &c, int ty, int ncu) {
     const int ncol = ceil_div(c.nx, kTX) * ceil_div(c.ny, ty);
     return ncu * ceil_div(ncol, ncu);
