@@ -154,21 +154,35 @@ def measure_batched(cg, steps: int, seq_steps: int, bundle: int = 10):
 
 
 def measure_case(cg, cfg, init, steps: int, storage=None, warmup: int = 5):
-    """One configuration run for `steps` timed steps from its initial state (autotuned tile)."""
+    """One configuration run for `steps` timed steps from its initial state, twice: with the tile
+    the autotuner picks on the initial state, and with 16-row tiles and planned chunks (which
+    re-cut themselves as the flow develops, something a tile chosen at step 0 cannot). Both runs
+    are reported; `value` is the faster."""
     storage = cg.STORE_F32 if storage is None else storage
-    with cg.Domain(cfg, storage=storage) as d:
-        init(d)
-        m0 = d.reduce()
-        tile = d.autotune(2)[:3]
-        d.step(warmup)
-        d.sync()
-        ms = d.time_steps(steps)
-        m1 = d.reduce()
+    runs = []
+    for planned in (False, True):
+        with cg.Domain(cfg, storage=storage) as d:
+            init(d)
+            m0 = d.reduce()
+            if planned:
+                tile = (16, 0, 2 if cfg.q == 19 else 1)
+                d.set_tile(*tile)
+            else:
+                tile = tuple(d.autotune(2)[:3])
+            d.step(warmup)
+            d.sync()
+            ms = d.time_steps(steps)
+            m1 = d.reduce()
+        runs.append((ms, tile, m0, m1))
     cells = cfg.interior_cells
+    ms, tile, m0, m1 = min(runs, key=lambda r: r[0])
     # relative per-phase mass change (None where a phase starts empty, as the jet's injected one)
     drift = [float((m1[k] - m0[k]) / m0[k]) if m0[k] > 0 else None for k in (0, 1)]
     return {"value": round(cells * steps / (ms / 1e3) / 1e6, 1), "unit": "MLUPS", "steps": steps,
-            "ms_total": round(ms, 2), "tile": tile, "bad_cells": int(m1[5]), "mass_drift_rel": drift}
+            "ms_total": round(ms, 2), "tile": list(tile), "bad_cells": int(m1[5]), "mass_drift_rel": drift,
+            "runs": [{"tile": list(t), "how": "planned chunks" if t[1] == 0 and i == 1 else "autotuned at step 0",
+                      "ms_total": round(r, 2), "value": round(cells * steps / (r / 1e3) / 1e6, 1)}
+                     for i, (r, t, _, _) in enumerate(runs)]}
 
 
 def measure_configs(cg):
