@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
@@ -169,6 +170,11 @@ int cglbm_domain_create(const cglbm_config *cfg, int32_t storage, cglbm_domain *
     if (e == hipSuccess) e = hipMalloc(&d->d_result, sizeof(double) * 8);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&d->plan.ncu, hipDeviceAttributeMultiprocessorCount, d->device);
     if (e == hipSuccess) {
+        // steps between re-plans; CGLBM_PLAN_PERIOD (1..100000) overrides for tuning sweeps
+        if (const char *pp = std::getenv("CGLBM_PLAN_PERIOD")) {
+            const long v = std::strtol(pp, nullptr, 10);
+            if (v >= 1 && v <= 100000) d->plan.period = static_cast<int>(v);
+        }
         d->plan.cap = plan_capacity(*cfg, d->plan.ncu);
         e = hipMalloc(&d->plan.items, sizeof(int4) * d->plan.cap);
     }
