@@ -205,17 +205,19 @@ int plan_items_for(const cglbm_config &c, int ty, int ncu) {
 }
 
 // Untuned shape of a library-owned domain: 16-row tiles with planned chunks (two exchange
-// buffers where they fit the LDS) when the plan fits the planner and its chunks average at least
-// 2 planes; else the size-aware uniform default. Measured on drops from 64^3 to 256^3, planned
-// chunks (with the measured correction) are as fast as the best uniform shape at 64^3 and 4-30%
-// faster from 96^3 up.
+// buffers where they fit the LDS) when the plan fits the planner, needs a single round of
+// work-groups and its chunks average at least 2 planes; else the size-aware uniform default.
+// Measured on drops from 64^3 to 256^3, planned chunks (with the measured correction) are as fast
+// as the best uniform shape at 64^3 and 4-30% faster from 96^3 up; plans of several rounds were
+// slower than uniform chunks (384^3: 19.6 against 28.4 G updates/s), and at 320^3 (one round,
+// with the planner given more LDS) planning gained nothing.
 MarchShape untuned_shape(const cglbm_config &c, const MarchPlanState *plan, const MarchShape &dflt) {
     if (!plan || plan->ncu < 1) return dflt;
     // a shape forced through the environment (march_default_shape) wins over planning
     if (std::getenv("CGLBM_MARCH_TY") || std::getenv("CGLBM_MARCH_L") || std::getenv("CGLBM_MARCH_NBUF")) return dflt;
     const int ntx = ceil_div(c.nx, kTX), nty = ceil_div(c.ny, 16);
     const int nitems = plan_items_for(c, 16, plan->ncu);
-    if (nitems > plan->cap || !plan_fits(ntx, nty, c.nz, nitems)) return dflt;
+    if (nitems > plan->cap || nitems > plan->ncu || !plan_fits(ntx, nty, c.nz, nitems)) return dflt;
     if (static_cast<int64_t>(ntx) * nty * c.nz < 2 * static_cast<int64_t>(nitems)) return dflt;
     return MarchShape{16, 0, c.q == 27 ? 1 : 2};
 }

@@ -140,13 +140,16 @@ def test_planned_equals_uniform_at_128(q):
         assert np.array_equal(states[0], states[1])
 
 
-@pytest.mark.parametrize("n,planned", [(48, False), (64, True), (128, True), (256, True), (320, False)])
-def test_default_plans_where_it_fits(n, planned):
-    """The untuned default plans its chunks when they average at least 2 planes (not at 48^3) and
-    the planner's LDS holds the domain's column tiles (not at 320^3, 200 column tiles of 320
-    planes); otherwise it keeps the size-aware uniform shape."""
+@pytest.mark.parametrize("shape,planned", [((48, 48, 48), False), ((64, 64, 64), True), ((128, 128, 128), True),
+                                          ((256, 256, 256), True), ((320, 320, 320), False), ((512, 272, 8), False)])
+def test_default_plans_where_it_fits(shape, planned):
+    """The untuned default plans its chunks when they average at least 2 planes (not at 48^3), the
+    planner's LDS holds the domain's column tiles (not at 320^3, 200 column tiles of 320 planes)
+    and one round of work-groups covers them (not for 272 column tiles); otherwise it keeps the
+    size-aware uniform shape."""
     cg = load_product()
-    cfg = cg.Config(nx=n, ny=n, nz=n, q=19)
+    n = min(shape)
+    cfg = cg.Config(nx=shape[0], ny=shape[1], nz=shape[2], q=19)
     with cg.Domain(cfg) as dom:
         dom.init_bubble(radius=n / 5.0, width=4.0)
         dom.step(3)
