@@ -84,6 +84,18 @@ def test_domain_create_without_gpu_fails_cleanly():
         cg.Server()
 
 
+def test_domain_entry_points_reject_null_domains():
+    """Domain calls given no domain return CGLBM_EINVAL with a message, without touching a GPU."""
+    import ctypes
+    cg = load_product()
+    L = cg.lib()
+    n, ty = ctypes.c_int32(), ctypes.c_int32()
+    assert L.cglbm_domain_plan(None, None, 0, ctypes.byref(n), ctypes.byref(ty)) == cg.EINVAL
+    assert "null domain" in L.cglbm_last_error().decode()
+    assert L.cglbm_domain_set_tile(None, 16, 0, 2) == cg.EINVAL
+    assert L.cglbm_step(None, 1) == cg.EINVAL
+
+
 def test_server_struct_layout_matches_header():
     """The ctypes mirrors of cglbm_request / cglbm_result / cglbm_server_opts / cglbm_server_stats
     have the sizes the C compiler gives the header's structs (checked with the host compiler)."""
