@@ -104,6 +104,29 @@ int cglbm_compact_layout(const cglbm_config *cfg, cglbm_layout *out);
 int cglbm_launch_rcs(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
                      const cglbm_layout *layout, int64_t step, int32_t kernel, void *stream);
 
+/* The z-marching kernel (the fast path) needs each state's 11 fields at one stride from one base
+ * (field f at A->rhor + f * stride, e.g. one allocation of 11 * stride floats split into the
+ * eleven pointers); fields in separate allocations take the layout-agnostic gather kernel, with
+ * the same results. */
+
+/* Balanced chunk plan for launch-level calls: what a library-owned domain keeps for itself
+ * (cglbm_domain_set_tile with L = 0), held by the caller for one pair of caller-owned states.
+ * No counterpart in the reference (its RCSKernel launch covers the domain uniformly). */
+typedef struct cglbm_plan cglbm_plan;
+int cglbm_plan_create(const cglbm_config *cfg, cglbm_plan **out);
+int cglbm_plan_destroy(cglbm_plan *plan);
+/* Forget the current cut (the caller replaced the state): the next launch records afresh. */
+int cglbm_plan_reset(cglbm_plan *plan);
+/* cglbm_launch_rcs on the z-marching kernel with 16-row tiles and planned chunks: every 64
+ * launches (and the first two after creation or a reset) the step records which waves took the
+ * interface form and how long each work-group took, and a one-work-group planner re-cuts the
+ * z-chunks so every CU gets equal work. Same results as cglbm_launch_rcs, bit for bit. Call it
+ * for consecutive steps of the same pair of states, in step order, on one stream; cfg must have
+ * the plan's extents and stencil. Where planning does not apply (fields not at one stride, more
+ * column tiles than one round of work-groups, or a planner limit) it runs as cglbm_launch_rcs. */
+int cglbm_launch_rcs_planned(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
+                             const cglbm_layout *layout, int64_t step, cglbm_plan *plan, void *stream);
+
 /* Boundary planes only: JetBoundaryConditions(B, Aold) (kernels.cu:102-114) or the wall copies.
  * B == Aold is allowed (main.cu:26). */
 int cglbm_launch_ybc(const cglbm_moments *B, const cglbm_moments *Aold, const cglbm_config *cfg,

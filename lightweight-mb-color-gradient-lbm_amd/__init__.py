@@ -36,6 +36,7 @@ EXPORTS = (
     "cglbm_domain_create", "cglbm_domain_destroy", "cglbm_domain_moments", "cglbm_domain_config",
     "cglbm_domain_bytes", "cglbm_domain_set_kernel", "cglbm_domain_set_tile", "cglbm_domain_autotune",
     "cglbm_domain_step_count", "cglbm_domain_stream", "cglbm_domain_plan",
+    "cglbm_plan_create", "cglbm_plan_destroy", "cglbm_plan_reset", "cglbm_launch_rcs_planned",
     "cglbm_init_bubble", "cglbm_init_jet", "cglbm_init_rti", "cglbm_step", "cglbm_time_steps", "cglbm_sync",
     "cglbm_copy_out", "cglbm_copy_in", "cglbm_reduce", "cglbm_write_vti", "cglbm_write_vti_host",
     "cglbm_batch_create", "cglbm_batch_destroy", "cglbm_batch_add", "cglbm_batch_remove", "cglbm_batch_size",
@@ -173,6 +174,10 @@ def lib():
         "cglbm_compact_layout": ([C, LP], ctypes.c_int),
         "cglbm_launch_rcs": ([M, M, C, LP, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p], ctypes.c_int),
         "cglbm_launch_ybc": ([M, M, C, LP, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
+        "cglbm_plan_create": ([C, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+        "cglbm_plan_destroy": ([ctypes.c_void_p], ctypes.c_int),
+        "cglbm_plan_reset": ([ctypes.c_void_p], ctypes.c_int),
+        "cglbm_launch_rcs_planned": ([M, M, C, LP, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
         "cglbm_domain_create": ([C, ctypes.c_int32, ctypes.POINTER(D)], ctypes.c_int),
         "cglbm_domain_destroy": ([D], ctypes.c_int),
         "cglbm_domain_moments": ([D, ctypes.c_int32, M], ctypes.c_int),
@@ -248,6 +253,45 @@ def launch_rcs(A: Moments, B: Moments, cfg: Config, layout: Layout | None = None
     L = layout if layout is not None else compact_layout(cfg)
     _check(lib().cglbm_launch_rcs(ctypes.byref(A), ctypes.byref(B), ctypes.byref(cc), ctypes.byref(L), step, kernel,
                                   stream), "cglbm_launch_rcs")
+
+
+class Plan:
+    """A caller-held chunk plan (cglbm_plan_*) for launch-level calls on one pair of states."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        _check(lib().cglbm_plan_create(ctypes.byref(cfg.c()), ctypes.byref(h)), "cglbm_plan_create")
+        self._h = h
+
+    def reset(self):
+        _check(lib().cglbm_plan_reset(self._h), "cglbm_plan_reset")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cglbm_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def launch_rcs_planned(A: Moments, B: Moments, cfg: Config, plan: Plan, layout: Layout | None = None, step: int = 1,
+                       stream: int | None = None):
+    """``cglbm_launch_rcs`` with the chunk plan `plan` (balanced z-chunks, same results)."""
+    cc = cfg.c()
+    L = layout if layout is not None else compact_layout(cfg)
+    _check(lib().cglbm_launch_rcs_planned(ctypes.byref(A), ctypes.byref(B), ctypes.byref(cc), ctypes.byref(L), step,
+                                          plan._h, stream), "cglbm_launch_rcs_planned")
 
 
 def launch_ybc(B: Moments, Aold: Moments, cfg: Config, layout: Layout | None = None, step: int = 0,

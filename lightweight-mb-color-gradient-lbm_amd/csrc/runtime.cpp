@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <fstream>
 #include <iomanip>
 #include <array>
@@ -334,6 +335,13 @@ int launch_step(const cglbm_moments *A, const cglbm_moments *B, const cglbm_conf
 
 }  // namespace cglbm
 
+// A caller-held chunk plan (cglbm_plan_create): the plan state of one pair of caller-owned states.
+struct cglbm_plan {
+    cglbm_config cfg{};
+    int device = 0;
+    cglbm::MarchPlanState st;
+};
+
 // ------------------------------------------------------------------------------------------------
 extern "C" {
 
@@ -362,6 +370,62 @@ int cglbm_launch_rcs(const cglbm_moments *A, const cglbm_moments *B, const cglbm
     if (A->rhor == B->rhor) return fail(CGLBM_EINVAL, "A and B alias");
     if (kernel < CGLBM_KERNEL_AUTO || kernel > CGLBM_KERNEL_MARCH) return fail(CGLBM_EINVAL, "unknown kernel %d", kernel);
     return launch_step(A, B, cfg, layout, step, kernel, nullptr, static_cast<hipStream_t>(stream));
+}
+
+int cglbm_plan_create(const cglbm_config *cfg, cglbm_plan **out) {
+    if (int rc = validate_config(cfg)) return rc;
+    if (!out) return fail(CGLBM_EINVAL, "null output");
+    *out = nullptr;
+    cglbm_plan *p = new (std::nothrow) cglbm_plan;
+    if (!p) return fail(CGLBM_ENOMEM, "host allocation failed");
+    p->cfg = *cfg;
+    hipError_t e = hipGetDevice(&p->device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&p->st.ncu, hipDeviceAttributeMultiprocessorCount, p->device);
+    if (e == hipSuccess) {
+        p->st.cap = plan_capacity(*cfg, p->st.ncu);
+        e = hipMalloc(&p->st.items, sizeof(int4) * p->st.cap);
+    }
+    if (e == hipSuccess) e = hipMalloc(&p->st.wmap, plan_wmap_bytes(*cfg));
+    if (e == hipSuccess) e = hipMalloc(&p->st.itime, sizeof(uint32_t) * p->st.cap);
+    if (e != hipSuccess) {
+        (void)cglbm_plan_destroy(p);
+        return hip_fail(e, "plan setup");
+    }
+    *out = p;
+    return CGLBM_OK;
+}
+
+int cglbm_plan_destroy(cglbm_plan *plan) {
+    if (!plan) return CGLBM_OK;
+    if (plan->st.items) (void)hipFree(plan->st.items);
+    if (plan->st.wmap) (void)hipFree(plan->st.wmap);
+    if (plan->st.itime) (void)hipFree(plan->st.itime);
+    delete plan;
+    return CGLBM_OK;
+}
+
+int cglbm_plan_reset(cglbm_plan *plan) {
+    if (!plan) return fail(CGLBM_EINVAL, "null plan");
+    plan->st.ty = 0;
+    return CGLBM_OK;
+}
+
+int cglbm_launch_rcs_planned(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
+                             const cglbm_layout *layout, int64_t step, cglbm_plan *plan, void *stream) {
+    if (!plan) return fail(CGLBM_EINVAL, "null plan");
+    if (int rc = validate_config(cfg)) return rc;
+    if (cfg->nx != plan->cfg.nx || cfg->ny != plan->cfg.ny || cfg->nz != plan->cfg.nz || cfg->q != plan->cfg.q)
+        return fail(CGLBM_EINVAL, "the plan was made for another domain shape or stencil");
+    if (int rc = check_layout(cfg, layout)) return rc;
+    if (int rc = check_moments(A, "A")) return rc;
+    if (int rc = check_moments(B, "B")) return rc;
+    if (A->rhor == B->rhor) return fail(CGLBM_EINVAL, "A and B alias");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    // plans of several rounds of work-groups are slower than uniform chunks: plan one round only
+    if (plan_items(*cfg, 16, plan->st.ncu) > plan->st.ncu)
+        return launch_step(A, B, cfg, layout, step, CGLBM_KERNEL_AUTO, nullptr, s);
+    const MarchShape m{16, 0, cfg->q == 27 ? 1 : 2};
+    return launch_step(A, B, cfg, layout, step, CGLBM_KERNEL_AUTO, &m, s, 0, 0, &plan->st);
 }
 
 // launch_JetBoundaryConditions (launch.cuh:78-83).

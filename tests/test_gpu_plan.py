@@ -154,3 +154,59 @@ def test_default_plans_where_it_fits(shape, planned):
         dom.init_bubble(radius=n / 5.0, width=4.0)
         dom.step(3)
         assert (dom.plan()[1].shape[0] > 0) == planned
+
+
+def test_launch_level_planned_matches_oracle(oracle):
+    """cglbm_launch_rcs_planned on caller-owned arrays in the reference's ghost-column layout (jet),
+    70 steps (a re-plan included): bitwise the oracle's result."""
+    cg = load_product()
+    p = oracle.Params(nx=40, ny=36, nz=30, q=19, ybc=oracle.Y_JET, jet_radius=5.0)
+    NX, NY, NZ = p.nx + 2, p.ny + 2, p.nz + 2
+    A0 = oracle.init_jet(p)
+    steps = 70
+    want = oracle.run(p, A0, steps)
+    ref = np.zeros((11, NZ, NY, NX), dtype=np.float32)
+    ref[:, 1:-1, :, 1:-1] = A0
+    cfg = _cfg(cg, p)
+    L = cg.Layout(1 + NX * NY, NX, NX * NY)
+    with cg.Domain(cg.Config(nx=NX, ny=NY, nz=NZ)) as carrier, cg.Plan(cfg) as plan:
+        carrier.set_state(ref)
+        bufs = [carrier.moments(0), carrier.moments(1)]
+        stream = cg.lib().cglbm_domain_stream(carrier._h)
+        for s in range(1, steps + 1):
+            cg.launch_rcs_planned(bufs[(s - 1) % 2], bufs[s % 2], cfg, plan, L, step=s, stream=stream)
+        carrier.sync()
+        got = carrier.get_state(0x7FF)
+    # an even number of steps leaves the result in the carrier's current buffer
+    np.testing.assert_array_equal(got[:, 1:-1, :, 1:-1], want)
+
+
+def test_launch_level_planned_equals_uniform_at_128():
+    """Planned launch-level steps of a 128^3 drop give the bits of uniform launch-level steps."""
+    cg = load_product()
+    cfg = cg.Config(nx=128, ny=128, nz=128, q=19, tau=0.8, sigma=0.01, beta=0.7)
+    states = []
+    for planned in (False, True):
+        with cg.Domain(cfg) as carrier:
+            carrier.init_bubble(radius=24.0, width=4.0)
+            bufs = [carrier.moments(0), carrier.moments(1)]
+            stream = cg.lib().cglbm_domain_stream(carrier._h)
+            plan = cg.Plan(cfg) if planned else None
+            for s in range(1, 81):
+                if planned:
+                    cg.launch_rcs_planned(bufs[(s - 1) % 2], bufs[s % 2], cfg, plan, step=s, stream=stream)
+                else:
+                    cg.launch_rcs(bufs[(s - 1) % 2], bufs[s % 2], cfg, step=s, kernel=cg.KERNEL_MARCH, stream=stream)
+            carrier.sync()
+            states.append(carrier.get_state())
+            if plan:
+                plan.close()
+    assert np.array_equal(states[0], states[1])
+
+
+def test_plan_rejects_another_shape():
+    cg = load_product()
+    with cg.Domain(cg.Config(nx=32, ny=32, nz=32)) as carrier, cg.Plan(cg.Config(nx=32, ny=32, nz=16)) as plan:
+        bufs = [carrier.moments(0), carrier.moments(1)]
+        with pytest.raises(cg.CglbmError, match="another domain shape"):
+            cg.launch_rcs_planned(bufs[0], bufs[1], cg.Config(nx=32, ny=32, nz=32), plan, step=1)
