@@ -22,6 +22,9 @@
 //  * The y boundary planes of the jet/RTI cases are written by the lanes that finish rows 1 and
 //    ny, so a step is one launch.
 //  * Blocks are remapped so that each XCD gets a contiguous range of tiles.
+//  * A work-group's item is a column tile and a z-range: uniform chunks of the launch shape, or
+//    the entries of an explicit list (MarchArgs::items) that the planner (plan.hip) cuts to equal
+//    cost from the interface map and item timings a recording step (REC) writes.
 #pragma once
 
 #include <cstdlib>
