@@ -95,6 +95,9 @@ ORACLE_CASES = [
     ("jet_noise_q27", dict(nx=24, ny=40, nz=20, q=27, ybc=1, tau=0.55, sigma=0.005, beta=0.7, jet_radius=5.0,
                            jet_noise=0.01, seed=9, outlet=1), 12),
     ("rti_q19", dict(nx=24, ny=48, nz=20, q=19, ybc=2, tau=0.6, sigma=0.002, beta=0.7, gravity=1e-4, atwood=0.5), 12),
+    # odd extents in every direction (partial tiles in x and y), across a re-plan
+    ("odd_q19", dict(nx=45, ny=37, nz=23, q=19, tau=0.7, sigma=0.01, beta=0.7), 70),
+    ("odd_jet_q27", dict(nx=35, ny=29, nz=21, q=27, ybc=1, tau=0.6, sigma=0.005, beta=0.7, jet_radius=4.0), 66),
 ]
 
 
