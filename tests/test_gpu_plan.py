@@ -213,3 +213,51 @@ def test_plan_rejects_another_shape():
         bufs = [carrier.moments(0), carrier.moments(1)]
         with pytest.raises(cg.CglbmError, match="another domain shape"):
             cg.launch_rcs_planned(bufs[0], bufs[1], cg.Config(nx=32, ny=32, nz=32), plan, step=1)
+
+
+def _greedy_count(costs, e, T):
+    n, acc = 1, e
+    for w in costs:
+        if acc + w > T:
+            n, acc = n + 1, e + w
+        else:
+            acc += w
+    return n
+
+
+def test_first_plan_is_optimal_under_its_model(oracle):
+    """The first cut of a state (before any measured correction) is, within the planner's 0.1-plane
+    search tolerance, the best contiguous cut under its cost model: recomputed here from the
+    state's interface map (own wave w of a 16-row tile holds rows 2w, 2w+1; a plane costs
+    1 + alpha * (most interface waves on one SIMD) / 2 in 1/32 units, a chunk e more; alpha 1.0,
+    e 3.6 for D3Q19), the largest modelled chunk equals the smallest bound that 256 greedy chunks
+    can meet."""
+    cg = load_product()
+    n = 128
+    p = oracle.Params(nx=n, ny=n, nz=n, q=19, tau=0.8, sigma=0.01, beta=0.7)
+    with cg.Domain(_cfg(cg, p)) as dom:
+        dom.set_tile(16, 0, 2)
+        dom.init_bubble(radius=24.0, width=4.0)
+        s = dom.get_state(0x3)
+        dom.step(1)  # records the map and makes the first (uncorrected) cut
+        ty, items = dom.plan()
+    assert ty == 16
+    rr, rb = s[0], s[1]
+    inv = np.float32(1.0) / (rr + rb)
+    on = (np.float32(4.0) * rr * rb * inv * inv) > np.float32(1e-4)          # (z, y, x)
+    waves = on.reshape(n, n // 16, 8, 2, n // 32, 32).any(axis=(3, 5))        # (z, tile y, wave, tile x)
+    loads = np.stack([waves[:, :, s_].astype(int) + waves[:, :, s_ + 4] for s_ in range(4)], axis=0).max(axis=0)
+    cost = np.rint(32.0 * (1.0 + 1.0 * loads / 2.0)) / 32.0                    # (z, tile y, tile x)
+    e = 3.6
+    cols = {(tx, tyi): cost[:, tyi, tx].astype(np.float64) for tyi in range(n // 16) for tx in range(n // 32)}
+    busy = items[items[:, 2] < items[:, 3]]
+    worst = max(e + cols[(tx, tyi)][z0:z1].sum() for tx, tyi, z0, z1 in busy)
+    lo, hi = e, e + max(c.sum() for c in cols.values()) + 1.0
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if sum(_greedy_count(c, e, mid) for c in cols.values()) <= items.shape[0]:
+            hi = mid
+        else:
+            lo = mid
+    assert worst <= hi + 0.1 + 1e-3, (worst, hi)
+    assert busy.shape[0] <= items.shape[0]
