@@ -104,10 +104,11 @@ int cglbm_compact_layout(const cglbm_config *cfg, cglbm_layout *out);
 int cglbm_launch_rcs(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
                      const cglbm_layout *layout, int64_t step, int32_t kernel, void *stream);
 
-/* The z-marching kernel (the fast path) needs each state's 11 fields at one stride from one base
- * (field f at A->rhor + f * stride, e.g. one allocation of 11 * stride floats split into the
- * eleven pointers); fields in separate allocations take the layout-agnostic gather kernel, with
- * the same results. */
+/* The z-marching kernel (the fast path) takes any placement of the 11 fields: at one stride from
+ * one base (field f at A->rhor + f * stride, one allocation of 11 * stride floats) it addresses a
+ * state below 4 GiB as one buffer; fields in separate allocations, as the reference's
+ * allocate_device (memory.cuh:59-81) makes them, it addresses through one 64-bit pointer each, with
+ * the same results at 95-99% of the rate. */
 
 /* Balanced chunk plan for launch-level calls: what a library-owned domain keeps for itself
  * (cglbm_domain_set_tile with L = 0), held by the caller for one pair of caller-owned states.
@@ -122,8 +123,8 @@ int cglbm_plan_reset(cglbm_plan *plan);
  * interface form and how long each work-group took, and a one-work-group planner re-cuts the
  * z-chunks so every CU gets equal work. Same results as cglbm_launch_rcs, bit for bit. Call it
  * for consecutive steps of the same pair of states, in step order, on one stream; cfg must have
- * the plan's extents and stencil. Where planning does not apply (fields not at one stride, more
- * column tiles than one round of work-groups, or a planner limit) it runs as cglbm_launch_rcs. */
+ * the plan's extents and stencil. Where planning does not apply (more column tiles than one round
+ * of work-groups, or a planner limit) it runs as cglbm_launch_rcs. */
 int cglbm_launch_rcs_planned(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
                              const cglbm_layout *layout, int64_t step, cglbm_plan *plan, void *stream);
 

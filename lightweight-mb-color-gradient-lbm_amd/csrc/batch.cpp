@@ -122,7 +122,7 @@ int rebuild(cglbm_batch *b) {
             const bool ok = d->storage == CGLBM_STORE_F16 ? march_args_f16(sa, d->buf[0], d->buf[1], d->fstride, bd.args)
                                                           : march_args(sa, bd.args);
             if (!ok) return set_error(CGLBM_EINVAL, "batch member not addressable by the fast path");
-            if (bd.args.store == kStoreF32Big)
+            if (bd.args.store == kStoreF32Ptr)
                 return set_error(CGLBM_EINVAL, "batch members must stay below 4 GiB per state");
             bd.buf0 = d->buf[0];
             bd.buf1 = d->buf[1];

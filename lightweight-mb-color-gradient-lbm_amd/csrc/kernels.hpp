@@ -32,7 +32,11 @@ struct MarchArgs {
     float jet_x0, jet_z0, jet_r, jet_u, jet_noise;
     uint32_t seed;
     int64_t step;
-    int store;  // kStoreF32 or kStoreF16 (storage.hpp); spans and strides refer to that layout
+    int store;  // kStoreF32, kStoreF16 or kStoreF32Ptr (storage.hpp); spans and strides refer to that layout
+    // Field bases of both states, field f of A at Af[f] and of B at Bf[f]: what the pointer form
+    // (kStoreF32Ptr) addresses; the buffer forms use A0/B0 and the strides.
+    const float *Af[11];
+    float *Bf[11];
     // Optional work plan (single-domain launches): items[i] = {tile x, tile y, z0, z1}, one per
     // work-group (z0 == z1: idle). Null: uniform chunks of the launch shape's length.
     const int4 *items;
@@ -72,8 +76,10 @@ hipError_t launch_batch(const BatchLaunch &L, int q, hipStream_t s);
 // device base += n (one thread), ordered on the stream after a bundle of n steps
 hipError_t launch_advance(int64_t *base, int n, hipStream_t s);
 
-// Fills `m` (FP32 storage) when the 11 fields of each state sit at one stride (the domain layout,
-// or caller arrays carved from one allocation); false otherwise.
+// Fills `m` (FP32 storage): the buffer form when the 11 fields of each state sit at one stride
+// from one base and the state is below 4 GiB (the domain layout, or caller arrays carved from one
+// allocation), else the pointer form (fields in separate allocations, larger states). False when
+// a field holds 2^32 cells or more.
 bool march_args(const StepArgs &a, MarchArgs &m);
 // Same geometry and physics for an FP16-deviation state pair at field stride fs (storage.hpp).
 bool march_args_f16(const StepArgs &a, const float *A0, float *B0, int64_t fs, MarchArgs &m);
@@ -85,8 +91,8 @@ hipError_t launch_march_q19_f32(const MarchArgs &a, MarchShape m, hipStream_t s)
 hipError_t launch_march_q19_f16(const MarchArgs &a, MarchShape m, hipStream_t s);
 hipError_t launch_march_q27_f32(const MarchArgs &a, MarchShape m, hipStream_t s);
 hipError_t launch_march_q27_f16(const MarchArgs &a, MarchShape m, hipStream_t s);
-hipError_t launch_march_q19_f32big(const MarchArgs &a, MarchShape m, hipStream_t s);
-hipError_t launch_march_q27_f32big(const MarchArgs &a, MarchShape m, hipStream_t s);
+hipError_t launch_march_q19_f32ptr(const MarchArgs &a, MarchShape m, hipStream_t s);
+hipError_t launch_march_q27_f32ptr(const MarchArgs &a, MarchShape m, hipStream_t s);
 hipError_t launch_march(const StepArgs &a, int q, hipStream_t s);
 
 // Balanced chunk plan of one domain (plan.hip): from the interface map of a recorded step

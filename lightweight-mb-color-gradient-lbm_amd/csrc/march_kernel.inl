@@ -129,6 +129,9 @@ struct FieldBuf {
                                               0x00020000);
         fs4 = fs * 4u;
     }
+    // the state read (src) and the state written (dst) by a step
+    __device__ __forceinline__ static FieldBuf src(const MarchArgs &a) { return FieldBuf(a.A0, a.fsA, a.spanA); }
+    __device__ __forceinline__ static FieldBuf dst(const MarchArgs &a) { return FieldBuf(a.B0, a.fsB, a.spanB); }
     __device__ __forceinline__ float ld(int f, uint32_t id) const {
         if (ST == kStoreF16 && f >= 2) {
             const uint32_t soff = static_cast<uint32_t>(f + 2) * (fs4 >> 1);
@@ -161,17 +164,27 @@ struct FieldBuf {
     }
 };
 
-// FP32 state of 4 GiB or more: the same accesses through a 64-bit pointer (the field base is
-// uniform, the cell offset a 32-bit VGPR, so the loads keep the scalar-base plus vector-offset
-// form). Every access of the kernel stays inside the state, as with the buffer form.
+// FP32 state addressed through one 64-bit pointer per field (fields in separate allocations, or a
+// state of 4 GiB or more): the same accesses, field f at p[f] + cell. Each field base is uniform
+// and the cell offset a 32-bit VGPR, so the loads keep the scalar-base plus vector-offset form.
+// Every access of the kernel stays inside the fields, as with the buffer form.
 template <>
-struct FieldBuf<kStoreF32Big> {
-    float *p;
-    uint64_t fs;
-    __device__ __forceinline__ FieldBuf(const float *base, uint32_t fs_, uint32_t)
-        : p(const_cast<float *>(base)), fs(fs_) {}
-    __device__ __forceinline__ float ld(int f, uint32_t id) const { return p[static_cast<uint64_t>(f) * fs + id]; }
-    __device__ __forceinline__ void st(int f, uint32_t id, float v) const { p[static_cast<uint64_t>(f) * fs + id] = v; }
+struct FieldBuf<kStoreF32Ptr> {
+    float *p[11];
+    __device__ __forceinline__ static FieldBuf src(const MarchArgs &a) {
+        FieldBuf b;
+#pragma unroll
+        for (int f = 0; f < 11; ++f) b.p[f] = const_cast<float *>(a.Af[f]);
+        return b;
+    }
+    __device__ __forceinline__ static FieldBuf dst(const MarchArgs &a) {
+        FieldBuf b;
+#pragma unroll
+        for (int f = 0; f < 11; ++f) b.p[f] = a.Bf[f];
+        return b;
+    }
+    __device__ __forceinline__ float ld(int f, uint32_t id) const { return p[f][id]; }
+    __device__ __forceinline__ void st(int f, uint32_t id, float v) const { p[f][id] = v; }
     __device__ __forceinline__ Mom mom(uint32_t id) const {
         Mom m;
         m.rr = ld(0, id);
@@ -284,8 +297,8 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     const int x0 = tix * TX, r0 = tiy * TY;
     const int t = threadIdx.x;
     const Coeffs c = a.c;
-    const FieldBuf<ST> Abuf(a.A0, a.fsA, a.spanA);
-    const FieldBuf<ST> Bbuf(a.B0, a.fsB, a.spanB);
+    const FieldBuf<ST> Abuf = FieldBuf<ST>::src(a);
+    const FieldBuf<ST> Bbuf = FieldBuf<ST>::dst(a);
 
     // ---- role and source cell of this lane -----------------------------------------------
     // ring waves: RW == 2 -> rows, columns; RW == 4 -> rows (pair half 0), rows (half 1),

@@ -48,23 +48,32 @@ MarchShape march_default_shape(const StepArgs &a, int q) {
 }
 
 bool march_args(const StepArgs &a, MarchArgs &m) {
-    const int64_t fsA = a.A[1] - a.A[0], fsB = a.B[1] - a.B[0];
-    if (fsA <= 0 || fsB <= 0) return false;
-    for (int f = 2; f < 11; ++f)
-        if (a.A[f] - a.A[0] != fsA * f || a.B[f] - a.B[0] != fsB * f) return false;
-    // last addressed cell of a field, in elements
+    // last addressed cell of a field, in elements; cell offsets stay 32-bit
     const int64_t nyt = a.ybc == kYPeriodic ? a.ny : a.ny + 2;
     const int64_t last = (a.nx - 1) + int64_t(a.py) * (nyt - 1) + int64_t(a.pz) * (a.nz - 1);
+    if (last >= (int64_t(1) << 32)) return false;
+    // Fields at one stride from one base (one allocation per state) and a state below 4 GiB: the
+    // buffer form. Otherwise (fields in separate allocations, or a larger state): one 64-bit
+    // pointer per field.
+    const auto addr = [](const float *p) { return static_cast<int64_t>(reinterpret_cast<uintptr_t>(p)); };
+    const int64_t dA = addr(a.A[1]) - addr(a.A[0]), dB = addr(a.B[1]) - addr(a.B[0]);
+    bool uniform = dA > 0 && dB > 0 && dA % 4 == 0 && dB % 4 == 0 && dA / 4 < (int64_t(1) << 32) &&
+                   dB / 4 < (int64_t(1) << 32);
+    for (int f = 2; f < 11 && uniform; ++f)
+        uniform = addr(a.A[f]) - addr(a.A[0]) == dA * f && addr(a.B[f]) - addr(a.B[0]) == dB * f;
+    const int64_t fsA = uniform ? dA / 4 : 0, fsB = uniform ? dB / 4 : 0;
     const int64_t spanA = (fsA * 10 + last + 1) * 4, spanB = (fsB * 10 + last + 1) * 4;
-    // cell offsets and field strides stay 32-bit; a state of 4 GiB or more takes 64-bit pointers
-    if (last >= (int64_t(1) << 32) || fsA >= (int64_t(1) << 32) || fsB >= (int64_t(1) << 32)) return false;
-    const bool big = spanA >= (int64_t(1) << 32) || spanB >= (int64_t(1) << 32);
+    const bool buffer = uniform && spanA < (int64_t(1) << 32) && spanB < (int64_t(1) << 32);
     m.A0 = a.A[0];
     m.B0 = a.B[0];
     m.fsA = static_cast<uint32_t>(fsA);
     m.fsB = static_cast<uint32_t>(fsB);
-    m.spanA = big ? 0u : static_cast<uint32_t>(spanA);
-    m.spanB = big ? 0u : static_cast<uint32_t>(spanB);
+    m.spanA = buffer ? static_cast<uint32_t>(spanA) : 0u;
+    m.spanB = buffer ? static_cast<uint32_t>(spanB) : 0u;
+    for (int f = 0; f < 11; ++f) {
+        m.Af[f] = a.A[f];
+        m.Bf[f] = a.B[f];
+    }
     m.nx = a.nx;
     m.ny = a.ny;
     m.nz = a.nz;
@@ -82,7 +91,7 @@ bool march_args(const StepArgs &a, MarchArgs &m) {
     m.jet_noise = a.jet_noise;
     m.seed = a.seed;
     m.step = a.step;
-    m.store = big ? kStoreF32Big : kStoreF32;
+    m.store = buffer ? kStoreF32 : kStoreF32Ptr;
     m.items = nullptr;
     m.nitems = 0;
     m.wmap = nullptr;
@@ -118,9 +127,9 @@ hipError_t launch_march_shape(const StepArgs &a, int q, MarchShape m, hipStream_
 
 hipError_t launch_march_args(const MarchArgs &a, int q, MarchShape m, hipStream_t s) {
     if ((m.ty != 4 && m.ty != 8 && m.ty != 16) || m.L < 1 || (m.nbuf != 1 && m.nbuf != 2)) return hipErrorInvalidValue;
-    if (a.store == kStoreF32Big) {
-        if (q == 19) return launch_march_q19_f32big(a, m, s);
-        if (q == 27) return launch_march_q27_f32big(a, m, s);
+    if (a.store == kStoreF32Ptr) {
+        if (q == 19) return launch_march_q19_f32ptr(a, m, s);
+        if (q == 27) return launch_march_q27_f32ptr(a, m, s);
         return hipErrorInvalidValue;
     }
     if (q == 19) return a.store == kStoreF16 ? launch_march_q19_f16(a, m, s) : launch_march_q19_f32(a, m, s);
@@ -129,7 +138,7 @@ hipError_t launch_march_args(const MarchArgs &a, int q, MarchShape m, hipStream_
 }
 
 hipError_t launch_batch(const BatchLaunch &L, int q, hipStream_t s) {
-    if (L.store == kStoreF32Big) return hipErrorInvalidValue;  // batch members stay below 4 GiB per state
+    if (L.store == kStoreF32Ptr) return hipErrorInvalidValue;  // batch members stay below 4 GiB per state
     if (q == 19) return L.store == kStoreF16 ? launch_batch_q19_f16(L, s) : launch_batch_q19_f32(L, s);
     if (q == 27) return L.store == kStoreF16 ? launch_batch_q27_f16(L, s) : launch_batch_q27_f32(L, s);
     return hipErrorInvalidValue;

@@ -27,9 +27,10 @@
 namespace cglbm {
 
 enum : int { kStoreF32 = 0, kStoreF16 = 1 };
-// FP32 storage addressed with 64-bit pointers (a single-domain state of 4 GiB or more, which the
-// 32-bit buffer offsets of the marching kernel cannot span). Internal to the kernels.
-enum : int { kStoreF32Big = 2 };
+// FP32 storage addressed through one 64-bit pointer per field: fields in separate allocations (one
+// array per moment, as the reference allocates them), or a single-domain state of 4 GiB or more,
+// which the 32-bit buffer offsets of the marching kernel cannot span. Internal to the kernels.
+enum : int { kStoreF32Ptr = 2 };
 
 __host__ __device__ constexpr float f16_scale(int f) { return f < 5 ? 1024.0f : 4096.0f; }
 __host__ __device__ constexpr float f16_inv_scale(int f) { return f < 5 ? (1.0f / 1024.0f) : (1.0f / 4096.0f); }
