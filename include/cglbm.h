@@ -148,7 +148,7 @@ int64_t cglbm_domain_bytes(const cglbm_config *cfg, int32_t storage);
 int cglbm_domain_set_kernel(cglbm_domain *dom, int32_t kernel);
 /* Tile shape of the z-marching kernel: ty tile rows (4, 8 or 16; tiles are 32 cells wide),
  * L planes per z-chunk, nbuf LDS exchange buffers (2: one barrier per plane; 1: half the LDS,
- * two barriers). L = 0 asks for planned chunks: every 64 steps the step records which waves took
+ * two barriers; D3Q27 with 16 rows always runs 1, which does not fit twice). L = 0 asks for planned chunks: every 64 steps the step records which waves took
  * the interface form and a planner cuts each column tile's z range into chunks of equal modelled
  * cost, one work-group per CU (results are bitwise the same for every shape). ty = 0 returns to
  * automatic selection: the autotuned record, else planned 16-row chunks (when they average at

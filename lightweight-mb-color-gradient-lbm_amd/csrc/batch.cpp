@@ -73,7 +73,7 @@ void drop_graph(cglbm_batch *b) {
 }
 
 MarchShape class_shape(const cglbm_batch *b, const Class &c) {
-    if (b->forced_shape.ty > 0) return b->forced_shape;
+    if (b->forced_shape.ty > 0) return effective_shape(c.q, b->forced_shape);
     if (c.shape_set) return c.shape;
     // Untuned class defaults, from class-by-class tile sweeps of batched members on MI355X:
     // 16-row tiles for D3Q27 and periodic D3Q19, 4-row tiles for bounded-y D3Q19.
@@ -450,6 +450,7 @@ int cglbm_batch_autotune(cglbm_batch *b, int32_t steps_per_candidate) {
         for (int nbuf = 1; nbuf <= 2; ++nbuf)
             for (int ty : tys)
                 for (int L : Ls) {
+                    if (effective_shape(b->classes[ci].q, MarchShape{ty, L, nbuf}).nbuf != nbuf) continue;
                     b->classes[ci].shape = MarchShape{ty, L, nbuf};
                     b->classes[ci].shape_set = true;
                     if (int rc = rebuild(b)) return rc;

@@ -234,7 +234,7 @@ int cglbm_domain_set_tile(cglbm_domain *dom, int32_t ty, int32_t L, int32_t nbuf
     }
     if ((ty != 4 && ty != 8 && ty != 16) || L < 0 || L > dom->cfg.nz || (nbuf != 1 && nbuf != 2))
         return set_error(CGLBM_EINVAL, "tile rows must be 4, 8 or 16, 0 <= L <= nz, nbuf 1 or 2");
-    dom->shape = MarchShape{ty, L, nbuf};
+    dom->shape = effective_shape(dom->cfg.q, MarchShape{ty, L, nbuf});
     return CGLBM_OK;
 }
 
@@ -268,6 +268,7 @@ int cglbm_domain_autotune(cglbm_domain *dom, int32_t steps_per_candidate, int32_
         for (int L : Ls) {
             if (L > dom->cfg.nz) continue;
             const MarchShape m{ty, L, nbuf};
+            if (effective_shape(dom->cfg.q, m).nbuf != nbuf) continue;  // runs as another candidate
             // warm-up launches, then the timed ones; a planned candidate starts from a fresh plan
             // and warms up through its two planning steps (the map, then the measured
             // correction), so its timed steps are the steady state between re-plans

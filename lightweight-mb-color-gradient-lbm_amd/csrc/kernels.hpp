@@ -18,8 +18,15 @@ struct MarchShape {
     int L;     // planes per z-chunk
     int nbuf;  // LDS exchange buffers: 2 = one barrier per plane, 1 = two barriers, half the LDS
 };
+// The shape a launch runs: D3Q27 16-row tiles keep one exchange buffer (two exceed the 160 KB LDS
+// of a CU), whatever was asked.
+inline MarchShape effective_shape(int q, MarchShape m) {
+    if (q == 27 && m.ty == 16) m.nbuf = 1;
+    return m;
+}
 // Arguments of the marching kernel: both states as one base pointer each plus a field stride
-// (every field at base + f * stride), read and written through buffer resources.
+// (every field at base + f * stride), read and written through buffer resources; or, in the
+// pointer form, one base pointer per field.
 struct MarchArgs {
     const float *A0;
     float *B0;

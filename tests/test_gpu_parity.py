@@ -184,6 +184,7 @@ def test_autotune_keeps_state_and_parity(oracle):
         dom.set_state(A)
         ty, L, nb, ms = dom.autotune(2)
         assert ty in (4, 8, 16) and 0 <= L <= p.nz and nb in (1, 2) and ms > 0  # L = 0: planned chunks
+        assert not (ty == 16 and nb == 2)  # reports the shape it ran: D3Q27 16-row tiles keep one buffer
         np.testing.assert_array_equal(dom.get_state(), A)  # autotuning does not advance the state
         dom.step(10)
         assert _golden.field_shas(dom.get_state()) == list(z["sha_10"])
