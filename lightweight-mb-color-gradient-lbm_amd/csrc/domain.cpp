@@ -269,11 +269,12 @@ int cglbm_domain_autotune(cglbm_domain *dom, int32_t steps_per_candidate, int32_
             if (L > dom->cfg.nz) continue;
             const MarchShape m{ty, L, nbuf};
             if (effective_shape(dom->cfg.q, m).nbuf != nbuf) continue;  // runs as another candidate
-            // warm-up launches, then the timed ones; a planned candidate starts from a fresh plan
-            // and warms up through its two planning steps (the map, then the measured
-            // correction), so its timed steps are the steady state between re-plans
+            // warm-up launches (the first launch of a kernel pays a one-time cost), then the
+            // timed ones; a planned candidate starts from a fresh plan and warms up through its
+            // two planning steps (the map, then the measured correction) and one planned step,
+            // so its timed steps are the steady state between re-plans
             if (L == 0) dom->plan.ty = 0;
-            for (int w = 0; w < (L == 0 ? 2 : 1); ++w)
+            for (int w = 0; w < (L == 0 ? 3 : 1); ++w)
                 if (int rc = launch_step(&dom->view[dom->cur], &dom->view[nxt], &dom->cfg, &dom->layout,
                                          dom->steps + 1, CGLBM_KERNEL_MARCH, &m, dom->stream, dom->storage,
                                          dom->fstride, &dom->plan))

@@ -65,10 +65,6 @@ __host__ __device__ constexpr int nslots() {
     return s;
 }
 
-// The exchange planes carry a one-cell margin so a tile source can push every direction
-// without a bounds test (pushes that leave the tile land in the margin and are never read) --
-// except the 16-row D3Q19 tile, where dropping the margin (pushes bounds-tested instead) is what
-// lets two exchange buffers fit the 160 KB LDS, so each plane needs one barrier instead of two.
 // Ring waves per work-group. The 16-row tile runs four (8 own waves + 4 = 3 per SIMD): the row
 // and the column ring each split their direction pairs over two waves, so the halo work spreads
 // over all four SIMDs instead of loading two of them. Smaller tiles run two (rows, columns).
@@ -94,14 +90,52 @@ __host__ __device__ constexpr uint32_t mask_half(uint32_t m, int h) {
     return h == 0 ? lo : (m & ~lo);
 }
 
+// Exchange planes: per buffer, one TY x TX plane of (red, blue) populations for each direction
+// that leaves the column (a slot). Both layouts give every cell one step outside the tile a
+// location that no lane reads, so a tile source pushes every direction without a bounds test:
+//  * margin (M = 1): slot s is a (TY + 2) x (TX + 2) array, tile cell (x, y) at row y + 1,
+//    column x + 1;
+//  * packed (M = 0, the 16-row D3Q19 tile, where a full margin would not let two buffers fit the
+//    160 KB LDS): the slots are stacked rows of TX + 1 cells, the last a padding column, with one
+//    padding row before each slot and after the last and one padding cell at the start. A push
+//    one cell past x = TX - 1 lands in the row's padding column, one before x = 0 in the padding
+//    column of the row before, one row outside the slot in a padding row.
+// Ring sources (outside the tile) still test their pushes: some land two cells out.
 template <int Q, int TY>
 __host__ __device__ constexpr int margin() {
     return (Q == 19 && TY == 16) ? 0 : 1;
 }
+template <int Q, int TY>
+struct XferLayout {
+    static constexpr int M = margin<Q, TY>();
+    static constexpr int NS = nslots<Q>();
+    static constexpr int RS = M ? TX + 2 : TX + 1;  // row stride (cells)
+    static constexpr int SR = M ? TY + 2 : TY + 1;  // rows per slot
+    static constexpr int SIZE = M ? NS * SR * RS : 1 + (NS * SR + 1) * RS;
+    // cell (x, y) of slot s, for -1 <= x <= TX and -1 <= y <= TY
+    __host__ __device__ static constexpr int at(int s, int x, int y) { return (RS + 1) + s * SR * RS + y * RS + x; }
+};
+// Compile-time proof of the layout: every cell within one step of the tile maps inside the
+// buffer, and onto a tile cell exactly when it is one.
+template <int Q, int TY>
+constexpr bool xfer_layout_ok() {
+    using XL = XferLayout<Q, TY>;
+    for (int s = 0; s < XL::NS; ++s)
+        for (int y = -1; y <= TY; ++y)
+            for (int x = -1; x <= TX; ++x) {
+                const int i = XL::at(s, x, y);
+                if (i < 0 || i >= XL::SIZE) return false;
+                const int t = i - (XL::RS + 1);
+                const int r = t % (XL::SR * XL::RS);
+                const bool tile = t >= 0 && t / (XL::SR * XL::RS) < XL::NS && r / XL::RS < TY && r % XL::RS < TX;
+                if (tile != (x >= 0 && x < TX && y >= 0 && y < TY)) return false;
+            }
+    return true;
+}
 template <int Q, int TY, int NBUF>
 struct Shared {
-    static constexpr int M = margin<Q, TY>();
-    float2 xfer[NBUF][nslots<Q>()][TY + 2 * M][TX + 2 * M];
+    static_assert(xfer_layout_ok<Q, TY>(), "exchange layout");
+    float2 xfer[NBUF][XferLayout<Q, TY>::SIZE];
     float psi[kPsiSlots][TY + 4][TX + 4];
 };
 
@@ -470,23 +504,21 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
         // ---- sources ----------------------------------------------------------------------
         [[maybe_unused]] bool ion = false;  // this lane's source took the interface form (REC)
         if (srow >= 0) {
-            constexpr int MG = margin<Q, TY>();
+            using XL = XferLayout<Q, TY>;
+            constexpr int MG = XL::M;
             auto lds_push = [&](auto I, float fr, float fb) {
                 constexpr int i = decltype(I)::value;
                 if constexpr (cross_col(i)) {
                     const int txx = sx + kCx[i], tyy = sy + kCy[i];
                     if (txx >= -MG && txx < TX + MG && tyy >= -MG && tyy < TY + MG)
-                        sh.xfer[xb][slot_of(i)][tyy + MG][txx + MG] = make_float2(fr, fb);
+                        sh.xfer[xb][XL::at(slot_of(i), txx, tyy)] = make_float2(fr, fb);
                 }
             };
             auto own_sink = [&](auto I, float fr, float fb) {
                 constexpr int i = decltype(I)::value;
                 if constexpr (cross_col(i)) {
-                    constexpr int cx = kCx[i], cy = kCy[i];
-                    // own lanes sit inside the tile: only the sides the direction points to are tested
-                    const bool in = MG != 0 || ((cx >= 0 || sx > 0) && (cx <= 0 || sx < TX - 1) &&
-                                                (cy >= 0 || sy > 0) && (cy <= 0 || sy < TY - 1));
-                    if (in) sh.xfer[xb][slot_of(i)][sy + cy + MG][sx + cx + MG] = make_float2(fr, fb);
+                    // at most one cell outside the tile: a location nobody reads (XferLayout)
+                    sh.xfer[xb][XL::at(slot_of(i), sx + kCx[i], sy + kCy[i])] = make_float2(fr, fb);
                 } else if constexpr (kCz[i] == 0) {
                     HB.v[i] = make_float2(fr, fb);
                 } else if constexpr (kCz[i] > 0) {
@@ -571,7 +603,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
             unroll<0, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
                 if constexpr (cross_col(i)) {
-                    const float2 v = sh.xfer[xb][slot_of(i)][sy + margin<Q, TY>()][sx + margin<Q, TY>()];
+                    const float2 v = sh.xfer[xb][XferLayout<Q, TY>::at(slot_of(i), sx, sy)];
                     if constexpr (kCz[i] == 0) HB.v[i] = v;
                     else if constexpr (kCz[i] > 0) HC.v[i] = v;
                     else HA.v[i] = v;
