@@ -221,12 +221,20 @@ struct Iface {
 // underflow, take the IEEE division. A NaN y (b NaN) keeps the NaN. Three operations instead of
 // the ten of the IEEE division sequence; bitwise equal to a / b (checked against the IEEE
 // quotient for every significand of b and the operand ranges of the colour-gradient terms).
+// The tiny-numerator case is a wave-uniform branch that stays a branch: left to itself the
+// compiler evaluates the IEEE division on every call and selects, which costs more than the
+// division it replaces; the empty volatile asm cannot be speculated, so the division runs only in
+// a wave that has such a lane.
 __device__ __forceinline__ float div_shared(float a, float b, float y) {
     float q = a * y;
     const float r = fmaf(-q, b, a);
     q = fmaf(r, y, q);
     q = (a == 0.0f && y == y) ? a : q;
-    if (fabsf(a) < 0x1p-100f && a != 0.0f) q = a / b;
+    const bool tiny = fabsf(a) < 0x1p-100f && a != 0.0f;
+    if (__builtin_amdgcn_ballot_w64(tiny) != 0) {
+        asm volatile("");
+        if (tiny) q = a / b;
+    }
     return q;
 }
 
