@@ -208,10 +208,13 @@ int plan_items_for(const cglbm_config &c, int ty, int ncu) {
 // Untuned shape of a library-owned domain: 16-row tiles with planned chunks (two exchange
 // buffers where they fit the LDS) when the plan fits the planner, needs a single round of
 // work-groups and its chunks average at least 2 planes; else the size-aware uniform default.
-// Measured on drops from 64^3 to 256^3, planned chunks (with the measured correction) are as fast
-// as the best uniform shape at 64^3 and 4-30% faster from 96^3 up; plans of several rounds were
+// Measured on drops from 64^3 to 256^3, planned chunks (with the measured correction) are within
+// 2-6% of the best uniform shape at 64^3 and 1-31% faster from 96^3 up; plans of several rounds were
 // slower than uniform chunks (384^3: 19.6 against 28.4 G updates/s), and at 320^3 (one round,
-// with the planner given more LDS) planning gained nothing.
+// with the planner given more LDS) planning gained nothing. A single-round domain with two chunks
+// per column whose heavy columns are heavy along their whole length (the 128 x 512 x 128
+// Rayleigh-Taylor layer) is planned 19-23% slower than uniform chunks; the autotuner, which times
+// both, picks uniform chunks there.
 MarchShape untuned_shape(const cglbm_config &c, const MarchPlanState *plan, const MarchShape &dflt) {
     if (!plan || plan->ncu < 1) return dflt;
     // a shape forced through the environment (march_default_shape) wins over planning
