@@ -108,7 +108,7 @@ int cglbm_launch_rcs(const cglbm_moments *A, const cglbm_moments *B, const cglbm
  * one base (field f at A->rhor + f * stride, one allocation of 11 * stride floats) it addresses a
  * state below 4 GiB as one buffer; fields in separate allocations, as the reference's
  * allocate_device (memory.cuh:59-81) makes them, it addresses through one 64-bit pointer each, with
- * the same results at 95-99% of the rate. */
+ * the same results at 95-100% of the rate. */
 
 /* Balanced chunk plan for launch-level calls: what a library-owned domain keeps for itself
  * (cglbm_domain_set_tile with L = 0), held by the caller for one pair of caller-owned states.
