@@ -9,7 +9,8 @@
  *   constants.cuh:9-93 + -DLBM_D3Q19 (compile.sh:9-26) ... cglbm_config
  *   launch_RCS (launch.cuh:89-94) ........................ cglbm_launch_rcs
  *   launch_JetBoundaryConditions (launch.cuh:78-83) ...... cglbm_launch_ybc
- *   launch_InitStaticBubble/InitRTI/InitJet (launch.cuh:14-72) ... cglbm_init_*
+ *   launch_InitStaticBubble/InitRTI/InitJet (launch.cuh:14-72) ... cglbm_launch_init_* (caller
+ *                                                         arrays), cglbm_init_* (domains)
  *   allocate_device_memory / free_device_memory (memory.cuh:153-197) ... cglbm_domain_create/destroy
  *   swap_moments (memory.cuh:200-203) .................... inside cglbm_step
  *   copy_out_D2H (memory.cuh:209-217) .................... cglbm_copy_out
@@ -99,8 +100,9 @@ int cglbm_compact_layout(const cglbm_config *cfg, cglbm_layout *out);
 
 /* One fused update of every interior cell, A -> B (RCSKernel, kernels.cu:119-131). When
  * ybc != periodic the ghost planes of B are written in the same launch sequence, so this one
- * call replaces main.cu:41 and main.cu:43. `step` keys the inlet perturbation. A and B must
- * not alias. kernel = CGLBM_KERNEL_*. */
+ * call replaces main.cu:41 and main.cu:43. `step` keys the inlet perturbation. No field of B
+ * may overlap a field of A or another field of B over the cells the layout addresses
+ * (CGLBM_EINVAL). kernel = CGLBM_KERNEL_*. */
 int cglbm_launch_rcs(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
                      const cglbm_layout *layout, int64_t step, int32_t kernel, void *stream);
 
@@ -133,6 +135,24 @@ int cglbm_launch_rcs_planned(const cglbm_moments *A, const cglbm_moments *B, con
 int cglbm_launch_ybc(const cglbm_moments *B, const cglbm_moments *Aold, const cglbm_config *cfg,
                      const cglbm_layout *layout, int64_t step, void *stream);
 
+/* Initial conditions on caller-owned arrays: the states of cglbm_init_bubble / cglbm_init_jet /
+ * cglbm_init_rti written into the 11 arrays of A at `layout` (cells the layout does not address,
+ * such as the reference's ghost columns, are left untouched). Computed on the host in double,
+ * uploaded through a transient staging buffer and scattered on `stream`; synchronous, as the
+ * reference's launchers synchronise the device.
+ *   launch_InitStaticBubble (launch.cuh:14-20; bubble/bubbleIn.cuh:15-52) ... cglbm_launch_init_bubble
+ *       (sharp = 1 is the reference's sphere; sharp = 0 the diffuse tanh drop)
+ *   launch_InitJet (launch.cuh:52-72; jet/jetIn.cuh:16-60) + main.cu:26 ... cglbm_launch_init_jet
+ *       (ybc must be CGLBM_Y_JET; the inlet disk comes from the boundary planes, as main.cu:26)
+ *   launch_InitRTI (launch.cuh:42-46; rti/rtiIn.cuh:18-119) ........... cglbm_launch_init_rti
+ *       (with ybc = CGLBM_Y_WALLS the wall planes are written as InitRTIWalls does) */
+int cglbm_launch_init_bubble(const cglbm_moments *A, const cglbm_config *cfg, const cglbm_layout *layout, double x0,
+                             double y0, double z0, double radius, double width, int32_t sharp, void *stream);
+int cglbm_launch_init_jet(const cglbm_moments *A, const cglbm_config *cfg, const cglbm_layout *layout, void *stream);
+int cglbm_launch_init_rti(const cglbm_moments *A, const cglbm_config *cfg, const cglbm_layout *layout,
+                          double amplitude, double width, int32_t n_random_modes, double random_amplitude,
+                          uint32_t seed, void *stream);
+
 /* ---- Domain objects: library-owned ping-pong state (LbmDevice replacement) -------------- */
 
 typedef struct cglbm_domain cglbm_domain;
@@ -153,7 +173,9 @@ int cglbm_domain_set_kernel(cglbm_domain *dom, int32_t kernel);
  * cost, one work-group per CU (results are bitwise the same for every shape). ty = 0 returns to
  * automatic selection: the autotuned record, else planned 16-row chunks (when they average at
  * least 2 planes and the planner's LDS holds a byte per (column tile, plane): cubes from 64^3
- * to 256^3), else a size-aware uniform shape. */
+ * to 256^3), checked once per state against the size-aware uniform shape (three steps of each
+ * timed with events on the domain's stream, read without blocking; the faster is kept), else
+ * the size-aware uniform shape. */
 int cglbm_domain_set_tile(cglbm_domain *dom, int32_t ty, int32_t L, int32_t nbuf);
 /* Autotuner: times each candidate tile shape (planned chunks, L = 0, among them) for
  * steps_per_candidate updates from the current state (the state does not advance), keeps the
@@ -191,8 +213,9 @@ int cglbm_copy_out(cglbm_domain *dom, uint32_t field_mask, float *host);
 int cglbm_copy_in(cglbm_domain *dom, uint32_t field_mask, const float *host);
 
 /* FP64 diagnostics of the current state over interior cells, fixed summation order:
- * out[0] = sum rhor, out[1] = sum rhob, out[2..4] = sum rho*u, out[5] = count of non-finite or
- * negative-density cells. Synchronous. */
+ * out[0] = sum rhor, out[1] = sum rhob, out[2..4] = sum rho*u, out[5] = count of bad cells: a
+ * negative or non-finite phase density, a non-positive density, or any non-finite velocity or Pi
+ * component (with FP16 storage a saturated field decodes to inf and counts). Synchronous. */
 int cglbm_reduce(cglbm_domain *dom, double out[6]);
 
 /* ASCII VTI with rhor rhob ux uy uz of the current state over the stored extent

@@ -37,6 +37,7 @@ EXPORTS = (
     "cglbm_domain_bytes", "cglbm_domain_set_kernel", "cglbm_domain_set_tile", "cglbm_domain_autotune",
     "cglbm_domain_step_count", "cglbm_domain_stream", "cglbm_domain_plan",
     "cglbm_plan_create", "cglbm_plan_destroy", "cglbm_plan_reset", "cglbm_launch_rcs_planned",
+    "cglbm_launch_init_bubble", "cglbm_launch_init_jet", "cglbm_launch_init_rti",
     "cglbm_init_bubble", "cglbm_init_jet", "cglbm_init_rti", "cglbm_step", "cglbm_time_steps", "cglbm_sync",
     "cglbm_copy_out", "cglbm_copy_in", "cglbm_reduce", "cglbm_write_vti", "cglbm_write_vti_host",
     "cglbm_batch_create", "cglbm_batch_destroy", "cglbm_batch_add", "cglbm_batch_remove", "cglbm_batch_size",
@@ -174,6 +175,11 @@ def lib():
         "cglbm_compact_layout": ([C, LP], ctypes.c_int),
         "cglbm_launch_rcs": ([M, M, C, LP, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p], ctypes.c_int),
         "cglbm_launch_ybc": ([M, M, C, LP, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
+        "cglbm_launch_init_bubble": ([M, C, LP] + [ctypes.c_double] * 5 + [ctypes.c_int32, ctypes.c_void_p],
+                                     ctypes.c_int),
+        "cglbm_launch_init_jet": ([M, C, LP, ctypes.c_void_p], ctypes.c_int),
+        "cglbm_launch_init_rti": ([M, C, LP, ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_double,
+                                   ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
         "cglbm_plan_create": ([C, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
         "cglbm_plan_destroy": ([ctypes.c_void_p], ctypes.c_int),
         "cglbm_plan_reset": ([ctypes.c_void_p], ctypes.c_int),
@@ -300,6 +306,36 @@ def launch_ybc(B: Moments, Aold: Moments, cfg: Config, layout: Layout | None = N
     L = layout if layout is not None else compact_layout(cfg)
     _check(lib().cglbm_launch_ybc(ctypes.byref(B), ctypes.byref(Aold), ctypes.byref(cc), ctypes.byref(L), step,
                                   stream), "cglbm_launch_ybc")
+
+
+def launch_init_bubble(A: Moments, cfg: Config, layout: Layout | None = None, radius: float = 8.0,
+                       width: float = 4.0, sharp: bool = False, center=None, stream: int | None = None):
+    """``launch_InitStaticBubble`` (launch.cuh:14-20) on caller-owned arrays (sharp = the
+    reference's sphere, else the diffuse tanh drop); centre defaults to the domain middle."""
+    cc = cfg.c()
+    L = layout if layout is not None else compact_layout(cfg)
+    if center is None:
+        center = ((cfg.nx - 1) / 2.0, (cfg.ny - 1) / 2.0, (cfg.nz - 1) / 2.0)
+    _check(lib().cglbm_launch_init_bubble(ctypes.byref(A), ctypes.byref(cc), ctypes.byref(L), *center, radius, width,
+                                          int(sharp), stream), "cglbm_launch_init_bubble")
+
+
+def launch_init_jet(A: Moments, cfg: Config, layout: Layout | None = None, stream: int | None = None):
+    """``launch_InitJet`` (launch.cuh:52-72) plus main.cu:26's boundary call, on caller-owned arrays."""
+    cc = cfg.c()
+    L = layout if layout is not None else compact_layout(cfg)
+    _check(lib().cglbm_launch_init_jet(ctypes.byref(A), ctypes.byref(cc), ctypes.byref(L), stream),
+           "cglbm_launch_init_jet")
+
+
+def launch_init_rti(A: Moments, cfg: Config, layout: Layout | None = None, amplitude: float = 2.0,
+                    width: float = 4.0, n_random_modes: int = 0, random_amplitude: float = 0.0, seed: int = 0,
+                    stream: int | None = None):
+    """``launch_InitRTI`` (launch.cuh:42-46) on caller-owned arrays."""
+    cc = cfg.c()
+    L = layout if layout is not None else compact_layout(cfg)
+    _check(lib().cglbm_launch_init_rti(ctypes.byref(A), ctypes.byref(cc), ctypes.byref(L), amplitude, width,
+                                       n_random_modes, random_amplitude, seed, stream), "cglbm_launch_init_rti")
 
 
 def write_vti_host(path: str, fields) -> None:

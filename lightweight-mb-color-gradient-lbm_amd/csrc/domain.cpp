@@ -76,6 +76,8 @@ void release(cglbm_domain *d) {
     if (d->plan.items) (void)hipFree(d->plan.items);
     if (d->plan.wmap) (void)hipFree(d->plan.wmap);
     if (d->plan.itime) (void)hipFree(d->plan.itime);
+    for (auto &ev : d->plan.tev)
+        if (ev) (void)hipEventDestroy(ev);
     if (d->ev0) (void)hipEventDestroy(d->ev0);
     if (d->ev1) (void)hipEventDestroy(d->ev1);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -94,6 +96,7 @@ int apply_ybc_views(cglbm_domain *d, const cglbm_moments &m) {
 int upload_init(cglbm_domain *d, const std::vector<float> &host, bool bc) {
     const int64_t n = stored_cells(d->cfg);
     d->plan.ty = 0;  // a new state: the next planned step records its interface map afresh
+    d->plan.trial = kTrialNone;  // and planned against uniform chunks is timed again
     Staging st;
     const cglbm_moments *m = &d->view[d->cur];
     if (d->storage == CGLBM_STORE_F16) {
@@ -119,6 +122,114 @@ int check_dom(const cglbm_domain *d) {
 // A batch member is stepped on the batch's stream; synchronous member operations wait for it.
 int wait_batch(cglbm_domain *d) {
     if (d->batch_stream) HTRY(hipStreamSynchronize(d->batch_stream));
+    return CGLBM_OK;
+}
+
+
+// Host-built initial states (11 fields, compact layout, computed in double and rounded once).
+int host_bubble(const cglbm_config &c, double x0, double y0, double z0, double radius, double width, int32_t sharp,
+                std::vector<float> &h) {
+    if (!(radius > 0) || (!sharp && !(width > 0))) return set_error(CGLBM_EINVAL, "radius/width must be positive");
+    const int nyt = stored_ny(c);
+    const int yoff = c.ybc == CGLBM_Y_PERIODIC ? 0 : 1;
+    const int64_t n = stored_cells(c);
+    h.assign(11 * n, 0.0f);
+    for (int z = 0; z < c.nz; ++z)
+        for (int yy = 0; yy < nyt; ++yy)
+            for (int x = 0; x < c.nx; ++x) {
+                const int64_t id = x + static_cast<int64_t>(c.nx) * (yy + static_cast<int64_t>(nyt) * z);
+                const double dx = x - x0, dy = (yy - yoff) - y0, dz = z - z0;
+                if (sharp) {
+                    const float fx = static_cast<float>(dx), fy = static_cast<float>(dy), fz = static_cast<float>(dz);
+                    const float r2 = fx * fx + fy * fy + fz * fz;
+                    const float R = static_cast<float>(radius);
+                    const float inside = static_cast<float>(r2 <= R * R);
+                    h[id] = (1.0f - inside) * 1.0f;
+                    h[n + id] = inside * 1.0f;
+                } else {
+                    const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    const double phi = std::tanh(2.0 * (radius - r) / width);
+                    h[id] = static_cast<float>(0.5 * (1.0 - phi));
+                    h[n + id] = static_cast<float>(0.5 * (1.0 + phi));
+                }
+            }
+    return CGLBM_OK;
+}
+
+// Red fluid at rest everywhere (jetIn.cuh:16-32); the inlet disk comes from the boundary planes.
+int host_jet(const cglbm_config &c, std::vector<float> &h) {
+    if (c.ybc != CGLBM_Y_JET) return set_error(CGLBM_EINVAL, "jet initialisation needs ybc = CGLBM_Y_JET");
+    const int64_t n = stored_cells(c);
+    h.assign(11 * n, 0.0f);
+    for (int64_t i = 0; i < n; ++i) h[i] = 1.0f;
+    return CGLBM_OK;
+}
+
+int host_rti(const cglbm_config &c, double amplitude, double width, int32_t n_random_modes, double random_amplitude,
+             uint32_t seed, std::vector<float> &h) {
+    if (!(width > 0) || n_random_modes < 0 || n_random_modes > 64) return set_error(CGLBM_EINVAL, "bad RTI parameters");
+    const int nyt = stored_ny(c);
+    const bool per = c.ybc == CGLBM_Y_PERIODIC;
+    const double PI = 3.14159265358979323846;
+    const double Lx = c.nx, Lz = c.nz;
+    const double y0 = (nyt - 1) / 2.0 + (per ? 1.0 : 0.0);
+    std::vector<double> ma(n_random_modes), mb(n_random_modes), ph(n_random_modes);
+    uint64_t s = seed;
+    for (int m = 0; m < n_random_modes; ++m) {
+        s += 0x9E3779B97F4A7C15ull;
+        ma[m] = 1 + (mix64(s) % 8);
+        s += 0x9E3779B97F4A7C15ull;
+        mb[m] = 1 + (mix64(s) % 8);
+        s += 0x9E3779B97F4A7C15ull;
+        ph[m] = 2.0 * PI * (static_cast<double>(mix64(s) >> 11) * (1.0 / 9007199254740992.0));
+    }
+    const int64_t n = stored_cells(c);
+    h.assign(11 * n, 0.0f);
+    for (int z = 0; z < c.nz; ++z)
+        for (int x = 0; x < c.nx; ++x) {
+            const double xr = x + 1.0, zr = z + 1.0;
+            double yi = y0 + amplitude * std::cos(2.0 * PI * xr / Lx) * std::cos(2.0 * PI * zr / Lz);
+            for (int m = 0; m < n_random_modes; ++m)
+                yi += random_amplitude * std::cos(2.0 * PI * (ma[m] * xr / Lx + mb[m] * zr / Lz) + ph[m]);
+            for (int yy = 0; yy < nyt; ++yy) {
+                const double yr = yy + (per ? 1.0 : 0.0);
+                const double phi = std::tanh((yr - yi) / width);
+                const int64_t id = x + static_cast<int64_t>(c.nx) * (yy + static_cast<int64_t>(nyt) * z);
+                h[id] = static_cast<float>(0.5 * (1.0 - phi));
+                h[n + id] = static_cast<float>(0.5 * (1.0 + phi));
+            }
+        }
+    return CGLBM_OK;
+}
+
+int check_launch_init(const cglbm_moments *A, const cglbm_config *cfg, const cglbm_layout *layout) {
+    if (int rc = validate_config(cfg)) return rc;
+    if (int rc = check_layout(cfg, layout)) return rc;
+    if (!A) return set_error(CGLBM_EINVAL, "null moments");
+    float *const *p = reinterpret_cast<float *const *>(A);
+    for (int f = 0; f < 11; ++f)
+        if (!p[f]) return set_error(CGLBM_EINVAL, "null field pointer");
+    return CGLBM_OK;
+}
+
+// A host-built compact state into caller arrays at `layout` (staging upload + scatter), then, if
+// `bc` and y is bounded, the boundary planes computed from it in place (main.cu:26).
+int upload_to_layout(const cglbm_moments *A, const cglbm_config &c, const cglbm_layout &L, const std::vector<float> &host,
+                     bool bc, hipStream_t s) {
+    Staging st;
+    const int64_t n = stored_cells(c);
+    HTRY(hipMalloc(&st.p, sizeof(float) * 11 * static_cast<size_t>(n)));
+    HTRY(hipMemcpyAsync(st.p, host.data(), sizeof(float) * 11 * static_cast<size_t>(n), hipMemcpyHostToDevice, s));
+    FieldPtrs dst;
+    float *const *p = reinterpret_cast<float *const *>(A);
+    for (int f = 0; f < 11; ++f) dst.p[f] = p[f] + L.origin;
+    HTRY(launch_scatter_fields(st.p, dst, c.nx, stored_ny(c), c.nz, static_cast<uint32_t>(L.pitch_y),
+                               static_cast<uint32_t>(L.pitch_z), s));
+    if (bc && c.ybc != CGLBM_Y_PERIODIC) {
+        BcArgs b = make_bc_args(A, A, &c, &L, 0);
+        HTRY(launch_ybc(b, s));
+    }
+    HTRY(hipStreamSynchronize(s));
     return CGLBM_OK;
 }
 
@@ -180,6 +291,8 @@ int cglbm_domain_create(const cglbm_config *cfg, int32_t storage, cglbm_domain *
     }
     if (e == hipSuccess) e = hipMalloc(&d->plan.wmap, plan_wmap_bytes(*cfg));
     if (e == hipSuccess) e = hipMalloc(&d->plan.itime, sizeof(uint32_t) * d->plan.cap);
+    for (auto &ev : d->plan.tev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipMemsetAsync(d->buf[0], 0, bytes, d->stream);
     if (e == hipSuccess) e = hipMemsetAsync(d->buf[1], 0, bytes, d->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
@@ -312,31 +425,8 @@ int cglbm_init_bubble(cglbm_domain *dom, double x0, double y0, double z0, double
                       int32_t sharp) {
     if (int rc = check_dom(dom)) return rc;
     if (int rc = wait_batch(dom)) return rc;
-    if (!(radius > 0) || (!sharp && !(width > 0))) return set_error(CGLBM_EINVAL, "radius/width must be positive");
-    const cglbm_config &c = dom->cfg;
-    const int nyt = stored_ny(c);
-    const int yoff = c.ybc == CGLBM_Y_PERIODIC ? 0 : 1;
-    const int64_t n = stored_cells(c);
-    std::vector<float> h(11 * n, 0.0f);
-    for (int z = 0; z < c.nz; ++z)
-        for (int yy = 0; yy < nyt; ++yy)
-            for (int x = 0; x < c.nx; ++x) {
-                const int64_t id = x + static_cast<int64_t>(c.nx) * (yy + static_cast<int64_t>(nyt) * z);
-                const double dx = x - x0, dy = (yy - yoff) - y0, dz = z - z0;
-                if (sharp) {
-                    const float fx = static_cast<float>(dx), fy = static_cast<float>(dy), fz = static_cast<float>(dz);
-                    const float r2 = fx * fx + fy * fy + fz * fz;
-                    const float R = static_cast<float>(radius);
-                    const float inside = static_cast<float>(r2 <= R * R);
-                    h[id] = (1.0f - inside) * 1.0f;
-                    h[n + id] = inside * 1.0f;
-                } else {
-                    const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
-                    const double phi = std::tanh(2.0 * (radius - r) / width);
-                    h[id] = static_cast<float>(0.5 * (1.0 - phi));
-                    h[n + id] = static_cast<float>(0.5 * (1.0 + phi));
-                }
-            }
+    std::vector<float> h;
+    if (int rc = host_bubble(dom->cfg, x0, y0, z0, radius, width, sharp, h)) return rc;
     dom->steps = 0;
     if (dom->batch) batch_member_changed(dom->batch);
     return upload_init(dom, h, true);
@@ -346,10 +436,8 @@ int cglbm_init_bubble(cglbm_domain *dom, double x0, double y0, double z0, double
 int cglbm_init_jet(cglbm_domain *dom) {
     if (int rc = check_dom(dom)) return rc;
     if (int rc = wait_batch(dom)) return rc;
-    if (dom->cfg.ybc != CGLBM_Y_JET) return set_error(CGLBM_EINVAL, "jet initialisation needs ybc = CGLBM_Y_JET");
-    const int64_t n = stored_cells(dom->cfg);
-    std::vector<float> h(11 * n, 0.0f);
-    for (int64_t i = 0; i < n; ++i) h[i] = 1.0f;  // red at rest (jetIn.cuh:16-32)
+    std::vector<float> h;
+    if (int rc = host_jet(dom->cfg, h)) return rc;
     dom->steps = 0;
     if (dom->batch) batch_member_changed(dom->batch);
     return upload_init(dom, h, true);
@@ -361,42 +449,40 @@ int cglbm_init_rti(cglbm_domain *dom, double amplitude, double width, int32_t n_
                    double random_amplitude, uint32_t seed) {
     if (int rc = check_dom(dom)) return rc;
     if (int rc = wait_batch(dom)) return rc;
-    if (!(width > 0) || n_random_modes < 0 || n_random_modes > 64) return set_error(CGLBM_EINVAL, "bad RTI parameters");
-    const cglbm_config &c = dom->cfg;
-    const int nyt = stored_ny(c);
-    const bool per = c.ybc == CGLBM_Y_PERIODIC;
-    const double PI = 3.14159265358979323846;
-    const double Lx = c.nx, Lz = c.nz;
-    const double y0 = (nyt - 1) / 2.0 + (per ? 1.0 : 0.0);
-    std::vector<double> ma(n_random_modes), mb(n_random_modes), ph(n_random_modes);
-    uint64_t s = seed;
-    for (int m = 0; m < n_random_modes; ++m) {
-        s += 0x9E3779B97F4A7C15ull;
-        ma[m] = 1 + (mix64(s) % 8);
-        s += 0x9E3779B97F4A7C15ull;
-        mb[m] = 1 + (mix64(s) % 8);
-        s += 0x9E3779B97F4A7C15ull;
-        ph[m] = 2.0 * PI * (static_cast<double>(mix64(s) >> 11) * (1.0 / 9007199254740992.0));
-    }
-    const int64_t n = stored_cells(c);
-    std::vector<float> h(11 * n, 0.0f);
-    for (int z = 0; z < c.nz; ++z)
-        for (int x = 0; x < c.nx; ++x) {
-            const double xr = x + 1.0, zr = z + 1.0;
-            double yi = y0 + amplitude * std::cos(2.0 * PI * xr / Lx) * std::cos(2.0 * PI * zr / Lz);
-            for (int m = 0; m < n_random_modes; ++m)
-                yi += random_amplitude * std::cos(2.0 * PI * (ma[m] * xr / Lx + mb[m] * zr / Lz) + ph[m]);
-            for (int yy = 0; yy < nyt; ++yy) {
-                const double yr = yy + (per ? 1.0 : 0.0);
-                const double phi = std::tanh((yr - yi) / width);
-                const int64_t id = x + static_cast<int64_t>(c.nx) * (yy + static_cast<int64_t>(nyt) * z);
-                h[id] = static_cast<float>(0.5 * (1.0 - phi));
-                h[n + id] = static_cast<float>(0.5 * (1.0 + phi));
-            }
-        }
+    std::vector<float> h;
+    if (int rc = host_rti(dom->cfg, amplitude, width, n_random_modes, random_amplitude, seed, h)) return rc;
     dom->steps = 0;
     if (dom->batch) batch_member_changed(dom->batch);
-    return upload_init(dom, h, c.ybc == CGLBM_Y_WALLS);
+    return upload_init(dom, h, dom->cfg.ybc == CGLBM_Y_WALLS);
+}
+
+// The same initial states on caller-owned arrays (launch_InitStaticBubble, launch_InitJet,
+// launch_InitRTI of launch.cuh:14-72, which take a MomentsDevice): the host-built state is
+// uploaded to a staging area and scattered into the arrays at `layout`; cells the layout does not
+// address (the reference's ghost columns) are left as they are. Synchronous on `stream`, as the
+// reference's launchers synchronise the device.
+int cglbm_launch_init_bubble(const cglbm_moments *A, const cglbm_config *cfg, const cglbm_layout *layout, double x0,
+                             double y0, double z0, double radius, double width, int32_t sharp, void *stream) {
+    if (int rc = check_launch_init(A, cfg, layout)) return rc;
+    std::vector<float> h;
+    if (int rc = host_bubble(*cfg, x0, y0, z0, radius, width, sharp, h)) return rc;
+    return upload_to_layout(A, *cfg, *layout, h, true, static_cast<hipStream_t>(stream));
+}
+
+int cglbm_launch_init_jet(const cglbm_moments *A, const cglbm_config *cfg, const cglbm_layout *layout, void *stream) {
+    if (int rc = check_launch_init(A, cfg, layout)) return rc;
+    std::vector<float> h;
+    if (int rc = host_jet(*cfg, h)) return rc;
+    return upload_to_layout(A, *cfg, *layout, h, true, static_cast<hipStream_t>(stream));
+}
+
+int cglbm_launch_init_rti(const cglbm_moments *A, const cglbm_config *cfg, const cglbm_layout *layout,
+                          double amplitude, double width, int32_t n_random_modes, double random_amplitude,
+                          uint32_t seed, void *stream) {
+    if (int rc = check_launch_init(A, cfg, layout)) return rc;
+    std::vector<float> h;
+    if (int rc = host_rti(*cfg, amplitude, width, n_random_modes, random_amplitude, seed, h)) return rc;
+    return upload_to_layout(A, *cfg, *layout, h, cfg->ybc == CGLBM_Y_WALLS, static_cast<hipStream_t>(stream));
 }
 
 // main.cu:38-55: update, boundary planes, swap.
@@ -467,6 +553,7 @@ int cglbm_copy_in(cglbm_domain *dom, uint32_t field_mask, const float *host) {
     const int64_t n = stored_cells(dom->cfg);
     const bool f16 = dom->storage == CGLBM_STORE_F16;
     dom->plan.ty = 0;
+    dom->plan.trial = kTrialNone;
     Staging st;
     if (f16 && (field_mask & ~0x3u))
         if (int rc = make_staging(dom, st, 1)) return rc;

@@ -120,6 +120,13 @@ hipError_t launch_pack_f16(const float *src, int64_t xs, float *state, int64_t f
                            hipStream_t s);
 hipError_t launch_unpack_f16(const float *state, int64_t fs, float *dst, int64_t xs, int64_t n, int f0, int f1,
                              hipStream_t s);
+// Compact FP32 fields (nx * nyt * nz cells each, back to back from src) into 11 caller arrays at
+// cell x + y py + z pz (pointers with the layout origin applied).
+struct FieldPtrs {
+    float *p[11];
+};
+hipError_t launch_scatter_fields(const float *src, const FieldPtrs &dst, int nx, int nyt, int nz, uint32_t py,
+                                 uint32_t pz, hipStream_t s);
 
 // FP64 diagnostics: partial sums per block into `partials` (capacity kReduceBlocks * 6), then the
 // fixed-order final sum into `result` (6 doubles).

@@ -146,6 +146,32 @@ int check_layout(const cglbm_config *c, const cglbm_layout *L) {
     return CGLBM_OK;
 }
 
+// The update reads A while it writes B, so no field of B may share memory with any field of A, nor
+// with another field of B: every pair is compared over the cells the layout addresses.
+int check_disjoint(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *c, const cglbm_layout *L) {
+    const int64_t nyt = c->ybc == CGLBM_Y_PERIODIC ? c->ny : c->ny + 2;
+    const int64_t last = L->origin + (c->nx - 1) + L->pitch_y * (nyt - 1) + L->pitch_z * (c->nz - 1);
+    const uintptr_t lo_off = static_cast<uintptr_t>(L->origin) * sizeof(float);
+    const uintptr_t hi_off = static_cast<uintptr_t>(last + 1) * sizeof(float);
+    float *const *pa = reinterpret_cast<float *const *>(A);
+    float *const *pb = reinterpret_cast<float *const *>(B);
+    for (int fa = 0; fa < 11; ++fa)
+        for (int fb = 0; fb < 11; ++fb) {
+            const uintptr_t a0 = reinterpret_cast<uintptr_t>(pa[fa]) + lo_off, a1 = reinterpret_cast<uintptr_t>(pa[fa]) + hi_off;
+            const uintptr_t b0 = reinterpret_cast<uintptr_t>(pb[fb]) + lo_off, b1 = reinterpret_cast<uintptr_t>(pb[fb]) + hi_off;
+            if (a0 < b1 && b0 < a1)
+                return fail(CGLBM_EINVAL, "A and B alias or overlap (A field %d and B field %d share memory)", fa, fb);
+        }
+    // two fields of B at one place would take each other's stores
+    for (int f = 0; f < 11; ++f)
+        for (int g = f + 1; g < 11; ++g) {
+            const uintptr_t f0 = reinterpret_cast<uintptr_t>(pb[f]) + lo_off, f1 = reinterpret_cast<uintptr_t>(pb[f]) + hi_off;
+            const uintptr_t g0 = reinterpret_cast<uintptr_t>(pb[g]) + lo_off, g1 = reinterpret_cast<uintptr_t>(pb[g]) + hi_off;
+            if (f0 < g1 && g0 < f1) return fail(CGLBM_EINVAL, "B fields %d and %d overlap", f, g);
+        }
+    return CGLBM_OK;
+}
+
 int check_moments(const cglbm_moments *m, const char *name) {
     if (!m) return fail(CGLBM_EINVAL, "null %s", name);
     float *const *p = reinterpret_cast<float *const *>(m);
@@ -213,8 +239,8 @@ int plan_items_for(const cglbm_config &c, int ty, int ncu) {
 // slower than uniform chunks (384^3: 19.6 against 28.4 G updates/s), and at 320^3 (one round,
 // with the planner given more LDS) planning gained nothing. A single-round domain with two chunks
 // per column whose heavy columns are heavy along their whole length (the 128 x 512 x 128
-// Rayleigh-Taylor layer) is planned 19-23% slower than uniform chunks; the autotuner, which times
-// both, picks uniform chunks there.
+// Rayleigh-Taylor layer) is planned 19-23% slower than uniform chunks: launch_default times both
+// once and keeps the faster, and the autotuner, which times both, picks uniform chunks there.
 MarchShape untuned_shape(const cglbm_config &c, const MarchPlanState *plan, const MarchShape &dflt) {
     if (!plan || plan->ncu < 1) return dflt;
     // a shape forced through the environment (march_default_shape) wins over planning
@@ -263,6 +289,61 @@ hipError_t launch_march_planned(MarchArgs ma, const cglbm_config &c, MarchShape 
     }
     return hipSuccess;
 }
+// The step of a domain left to the default shape (no tile set, no autotuned record): planned chunks
+// where untuned_shape picks them, checked once against the uniform default by timing both on the
+// domain's stream (MarchPlanState::trial). Planned chunks lose where the heavy column tiles are
+// heavy along their whole length and the single round of work-groups cannot even them out (a flat
+// Rayleigh-Taylor layer across x and z); the hardware's own dispatch of many uniform work-groups
+// balances such a case better. The planned steps' share of the re-plans (a recording step and
+// the planner every `period` steps, about 2% of a step) is charged to them.
+hipError_t launch_default(const MarchArgs &ma, const cglbm_config &c, MarchPlanState *plan, const MarchShape &dflt,
+                          hipStream_t s) {
+    const MarchShape m = untuned_shape(c, plan, dflt);
+    if (m.L != 0 || !plan || !plan->tev[0]) return launch_march_planned(ma, c, m, plan, dflt.L, s);
+    hipError_t e = hipSuccess;
+    switch (plan->trial) {
+    case kTrialKeepUniform:
+        return launch_march_args(ma, c.q, dflt, s);
+    case kTrialUniform:
+        e = launch_march_args(ma, c.q, dflt, s);
+        if (e == hipSuccess && --plan->trial_left == 0) {
+            e = hipEventRecord(plan->tev[3], s);
+            plan->trial = kTrialWait;
+        }
+        return e;
+    case kTrialWait:
+        if (hipEventQuery(plan->tev[3]) != hipSuccess) {
+            // not complete yet: the query's status must not surface as the next launch's error
+            (void)hipGetLastError();
+        } else {
+            float tp = 0.0f, tu = 0.0f;
+            const bool ok = hipEventElapsedTime(&tp, plan->tev[0], plan->tev[1]) == hipSuccess &&
+                            hipEventElapsedTime(&tu, plan->tev[2], plan->tev[3]) == hipSuccess;
+            plan->trial = (ok && tu < tp * 1.02f) ? kTrialKeepUniform : kTrialKeepPlanned;
+            if (plan->trial == kTrialKeepUniform) return launch_march_args(ma, c.q, dflt, s);
+        }
+        break;
+    case kTrialNone:
+        // start on a corrected plan, with no re-plan due within the timed steps
+        if (plan->ty == m.ty && plan->since + kTrialSteps < plan->period) {
+            e = hipEventRecord(plan->tev[0], s);
+            if (e != hipSuccess) return e;
+            plan->trial = kTrialPlanned;
+            plan->trial_left = kTrialSteps;
+        }
+        break;
+    default:
+        break;
+    }
+    e = launch_march_planned(ma, c, m, plan, dflt.L, s);
+    if (e == hipSuccess && plan->trial == kTrialPlanned && --plan->trial_left == 0) {
+        e = hipEventRecord(plan->tev[1], s);
+        if (e == hipSuccess) e = hipEventRecord(plan->tev[2], s);
+        plan->trial = kTrialUniform;
+        plan->trial_left = kTrialSteps;
+    }
+    return e;
+}
 }  // namespace
 
 int plan_items(const cglbm_config &c, int ty, int ncu) { return plan_items_for(c, ty, ncu); }
@@ -303,9 +384,10 @@ int launch_step(const cglbm_moments *A, const cglbm_moments *B, const cglbm_conf
             return fail(CGLBM_EINVAL, "FP16 storage needs the z-marching kernel and a supported case");
         MarchShape m;
         const MarchShape dflt = march_default_shape(a, cfg->q);
-        if (shape && shape->ty > 0) m = *shape;
-        else if (!tuned_shape_lookup(*cfg, store, &m)) m = untuned_shape(*cfg, plan, dflt);
-        const hipError_t e = launch_march_planned(ma, *cfg, m, plan, dflt.L, s);
+        hipError_t e;
+        if (shape && shape->ty > 0) e = launch_march_planned(ma, *cfg, *shape, plan, dflt.L, s);
+        else if (tuned_shape_lookup(*cfg, store, &m)) e = launch_march_planned(ma, *cfg, m, plan, dflt.L, s);
+        else e = launch_default(ma, *cfg, plan, dflt, s);
         if (e != hipSuccess) return hip_fail(e, "launch_march (FP16 storage)");
         return CGLBM_OK;
     }
@@ -314,15 +396,15 @@ int launch_step(const cglbm_moments *A, const cglbm_moments *B, const cglbm_conf
         return fail(CGLBM_EINVAL, "the z-marching kernel does not support this shape/layout");
     hipError_t e;
     if (kernel != CGLBM_KERNEL_GATHER && march_ok) {
-        MarchShape m;
+        MarchShape m{0, 0, 0};
         const MarchShape dflt = march_default_shape(a, cfg->q);
         if (shape && shape->ty > 0) m = *shape;
-        else if (!tuned_shape_lookup(*cfg, store, &m)) m = untuned_shape(*cfg, plan, dflt);
-        if ((m.ty != 4 && m.ty != 8 && m.ty != 16) || m.L < 0 || (m.nbuf != 1 && m.nbuf != 2))
+        else if (!tuned_shape_lookup(*cfg, store, &m)) m = MarchShape{0, 0, 0};
+        if (m.ty != 0 && ((m.ty != 4 && m.ty != 8 && m.ty != 16) || m.L < 0 || (m.nbuf != 1 && m.nbuf != 2)))
             return fail(CGLBM_EINVAL, "bad marching shape");
         MarchArgs ma;
         if (!march_args(a, ma)) return fail(CGLBM_EINVAL, "the z-marching kernel does not support this shape/layout");
-        e = launch_march_planned(ma, *cfg, m, plan, dflt.L, s);
+        e = m.ty != 0 ? launch_march_planned(ma, *cfg, m, plan, dflt.L, s) : launch_default(ma, *cfg, plan, dflt, s);
         if (e != hipSuccess) return hip_fail(e, "launch_march");
         return CGLBM_OK;  // boundary planes are written inside the marching step
     }
@@ -370,7 +452,7 @@ int cglbm_launch_rcs(const cglbm_moments *A, const cglbm_moments *B, const cglbm
     if (int rc = check_layout(cfg, layout)) return rc;
     if (int rc = check_moments(A, "A")) return rc;
     if (int rc = check_moments(B, "B")) return rc;
-    if (A->rhor == B->rhor) return fail(CGLBM_EINVAL, "A and B alias");
+    if (int rc = check_disjoint(A, B, cfg, layout)) return rc;
     if (kernel < CGLBM_KERNEL_AUTO || kernel > CGLBM_KERNEL_MARCH) return fail(CGLBM_EINVAL, "unknown kernel %d", kernel);
     return launch_step(A, B, cfg, layout, step, kernel, nullptr, static_cast<hipStream_t>(stream));
 }
@@ -422,7 +504,7 @@ int cglbm_launch_rcs_planned(const cglbm_moments *A, const cglbm_moments *B, con
     if (int rc = check_layout(cfg, layout)) return rc;
     if (int rc = check_moments(A, "A")) return rc;
     if (int rc = check_moments(B, "B")) return rc;
-    if (A->rhor == B->rhor) return fail(CGLBM_EINVAL, "A and B alias");
+    if (int rc = check_disjoint(A, B, cfg, layout)) return rc;
     const hipStream_t s = static_cast<hipStream_t>(stream);
     // plans of several rounds of work-groups are slower than uniform chunks: plan one round only
     if (plan_items(*cfg, 16, plan->st.ncu) > plan->st.ncu)

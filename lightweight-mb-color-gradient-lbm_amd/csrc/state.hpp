@@ -31,7 +31,24 @@ struct MarchPlanState {
     int ty = 0;               // tile rows the current plan was made for; 0 = none yet
     int period = 64;
     int since = 0;            // steps since the plan was made
+    // Planned or uniform chunks, for a domain left to the default shape: once the first plan and
+    // its measured correction are made, kTrialSteps planned steps and kTrialSteps steps of the
+    // uniform default are timed with events on the domain's stream; the result is read without
+    // blocking when the events have completed, and the faster form is kept until the state is
+    // replaced. Results are bitwise the same either way. No events (caller plans): no trial.
+    hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int trial = 0;            // kTrial* below
+    int trial_left = 0;       // steps left in the current timed phase
 };
+enum : int {
+    kTrialNone = 0,     // not measured yet (planned steps meanwhile)
+    kTrialPlanned = 1,  // timing planned steps
+    kTrialUniform = 2,  // timing uniform steps
+    kTrialWait = 3,     // both timed, result pending (planned steps meanwhile)
+    kTrialKeepPlanned = 4,
+    kTrialKeepUniform = 5,
+};
+inline constexpr int kTrialSteps = 3;
 // Work items of a planned step for tiles of `ty` rows: one per CU, rounded up to a multiple of
 // the CU count so every column tile gets at least one.
 int plan_items(const cglbm_config &c, int ty, int ncu);

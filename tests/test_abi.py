@@ -70,6 +70,28 @@ def test_launch_rejects_bad_layouts_and_aliasing():
         cg.launch_rcs(A, nullB, cfg)
 
 
+def test_launch_rejects_overlapping_fields():
+    """Any field of B overlapping any field of A, or another field of B, over the addressed cells
+    is refused before a device call (the update would read A while writing B). Disjoint fields
+    pass the check: every GPU test that launches on caller arrays covers that side."""
+    cg = load_product()
+    cfg = cg.Config(nx=8, ny=8, nz=8)
+    n = 8 * 8 * 8 * 4  # bytes one field spans in the compact layout
+    base = 1 << 20
+    a = [base + f * n for f in range(11)]
+    b = [base + (11 + f) * n for f in range(11)]
+    # (A field moved onto, B field, byte shift, first overlapping A field in the check's order)
+    for fa, fb, shift, first in ((0, 0, 0, 0), (7, 3, 4, 7), (10, 10, n - 4, 10), (2, 9, -(n - 4), 1)):
+        bb = list(b)
+        bb[fb] = a[fa] + shift
+        with pytest.raises(cg.CglbmError, match=f"A field {first} and B field {fb}"):
+            cg.launch_rcs(cg.Moments(*a), cg.Moments(*bb), cfg)
+    bb = list(b)
+    bb[4] = b[8] + 4
+    with pytest.raises(cg.CglbmError, match="B fields 4 and 8 overlap"):
+        cg.launch_rcs(cg.Moments(*a), cg.Moments(*bb), cfg)
+
+
 def test_domain_create_without_gpu_fails_cleanly():
     cg = load_product()
     try:

@@ -72,6 +72,11 @@ CONFIGS = [
     ("rti_q19", dict(nx=24, ny=48, nz=20, q=19, ybc=2, tau=0.6, sigma=0.002, beta=0.7, gravity=1e-4, atwood=0.5),
      30),
     ("conservative_q19", dict(nx=20, ny=20, nz=20, q=19, conservative_omega2=1, sigma=0.02), 20),
+    # a body force on a periodic y axis: the automatic choice takes the gather kernel with forcing
+    ("forced_periodic_q19", dict(nx=16, ny=16, nz=16, q=19, tau=0.7, sigma=0.01, beta=0.7, gravity=1e-4,
+                                 atwood=0.5), 20),
+    ("forced_periodic_q27", dict(nx=16, ny=14, nz=12, q=27, tau=0.7, sigma=0.01, beta=0.7, gravity=1e-4,
+                                 atwood=0.5), 20),
 ]
 
 
@@ -123,7 +128,22 @@ def test_reduce_is_deterministic_and_flags_bad_cells(oracle):
         S = dom.get_state()
         S[0, 3, 4, 5] = np.nan
         S[1, 7, 8, 9] = -1.0
+        S[7, 10, 11, 12] = np.inf   # a Pi component
+        S[10, 3, 4, 5] = np.nan     # same cell as the first: still one bad cell
         dom.set_state(S)
+        assert dom.reduce()[5] == 3
+
+
+def test_reduce_counts_saturated_fp16_pi(oracle):
+    """FP16-deviation storage holds |Pi| < 16: a larger value saturates to inf on encoding and the
+    diagnostics count the cell as non-finite, like a saturated velocity."""
+    cg = load_product()
+    p = oracle.Params(nx=16, ny=16, nz=16, q=19)
+    A = oracle.init_bubble(p, radius=4.0)
+    A[9, 2, 3, 4] = 100.0     # Pizz beyond the binary16 range at its scale
+    A[3, 5, 6, 7] = 1000.0    # uy beyond the binary16 range at its scale
+    with cg.Domain(_cfg(cg, p), storage=cg.STORE_F16) as dom:
+        dom.set_state(A)
         assert dom.reduce()[5] == 2
 
 
@@ -150,6 +170,60 @@ def test_launch_level_reference_layout(oracle):
         carrier.sync()
         got = carrier.get_state()
     np.testing.assert_array_equal(got[:, 1:-1, :, 1:-1], want)
+
+
+# (label, oracle params, initialiser name, keyword arguments)
+INIT_CASES = [
+    ("bubble_tanh_q19", dict(nx=20, ny=18, nz=16, q=19), "bubble", dict(radius=5.0, width=4.0, sharp=False)),
+    ("bubble_sharp_q27", dict(nx=17, ny=20, nz=15, q=27), "bubble", dict(radius=5.0, width=4.0, sharp=True)),
+    ("bubble_offcentre_q19", dict(nx=18, ny=16, nz=20, q=19), "bubble",
+     dict(radius=4.5, width=3.0, sharp=False, center=(5.0, 9.5, 12.25))),
+    ("jet_q27", dict(nx=20, ny=18, nz=16, q=27, ybc=1, jet_radius=4.0), "jet", {}),
+    ("rti_walls_q19", dict(nx=16, ny=24, nz=12, q=19, ybc=2), "rti",
+     dict(amplitude=2.0, width=4.0, n_random_modes=8, random_amplitude=0.2, seed=5)),
+]
+
+
+@pytest.mark.parametrize("label,kw,kind,args", INIT_CASES, ids=[c[0] for c in INIT_CASES])
+def test_launch_level_initialisers_reference_layout(oracle, label, kw, kind, args):
+    """cglbm_launch_init_* (launch_InitStaticBubble / launch_InitJet / launch_InitRTI,
+    launch.cuh:14-72) on caller arrays in the reference's NX*NY*NZ ghost-column layout: the
+    addressed cells equal the oracle's initialisers bit for bit (whose own bits are pinned to the
+    reference's initialisers by the golden vectors) and every other cell keeps its previous
+    value."""
+    cg = load_product()
+    p = oracle.Params(**kw)
+    NX, NZ = p.nx + 2, p.nz + 2
+    NY = p.ny + 2
+    if kind == "bubble":
+        want = oracle.init_bubble(p, args["radius"], args["width"], args["sharp"], args.get("center"))
+    elif kind == "jet":
+        want = oracle.init_jet(p)
+    else:
+        want = oracle.init_rti(p, args["amplitude"], args["width"], args["n_random_modes"], args["random_amplitude"],
+                               args["seed"])
+    periodic = p.ybc == oracle.Y_PERIODIC
+    y0 = 1 if periodic else 0
+    L = cg.Layout(1 + NX * NY + (NX if periodic else 0), NX, NX * NY)
+    cfg = _cfg(cg, p)
+    fill = np.full((11, NZ, NY, NX), 7.5, dtype=np.float32)  # marks cells the call must not touch
+    with cg.Domain(cg.Config(nx=NX, ny=NY, nz=NZ)) as carrier:
+        carrier.set_state(fill)
+        A = carrier.moments(0)
+        stream = cg.lib().cglbm_domain_stream(carrier._h)
+        print(f"init {label}", flush=True)
+        if kind == "bubble":
+            cg.launch_init_bubble(A, cfg, L, stream=stream, **args)
+        elif kind == "jet":
+            cg.launch_init_jet(A, cfg, L, stream=stream)
+        else:
+            cg.launch_init_rti(A, cfg, L, stream=stream, **args)
+        got = carrier.get_state()
+    inner = got[:, 1:-1, y0:y0 + p.nyt, 1:-1]
+    np.testing.assert_array_equal(inner, want)
+    outside = got.copy()
+    outside[:, 1:-1, y0:y0 + p.nyt, 1:-1] = 7.5
+    assert np.all(outside == 7.5)
 
 
 SHAPES = [(4, 3, 1), (8, 5, 2), (16, 7, 1), (16, 16, 2), (8, 1, 2), (4, 100, 2), (16, 0, 2), (4, 0, 1)]
