@@ -201,38 +201,155 @@ def measure_configs(cg):
                    atwood=0.5)
     rti = lambda d: d.init_rti(2.0, 4.0, 8, 0.2, 0)
     out["rti_d3q19_128x512x128_f16"] = measure_case(cg, c3, rti, 5000, storage=cg.STORE_F16)
+    out["rti_d3q19_128x512x128_f16"]["error_vs_fp64"] = precision_table().get("rti_128x512x128_5000")
     out["rti_d3q19_128x512x128_f32"] = measure_case(cg, c3, rti, 5000)
+    out["bubble64_d3q19"]["mass_modes"] = mass_modes(cg)
     return out
 
 
-def traffic_from_profiles(workload: str):
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
-    if not os.path.exists(path):
-        return None
+def mass_modes(cg):
+    """Per-phase relative mass change of the 64^3 static bubble after 100 and 1000 steps, in the
+    reference's mode and in the conservative variant (no Omega2 where |F| <= 1e-4), from the FP64
+    diagnostics. Binary32 arithmetic grows the mass by about the binary32 weight sum's excess over
+    1 per step in both modes; the variant removes only the reference's Omega2 leak."""
+    out = {}
+    for cons in (0, 1):
+        cfg = cg.Config(nx=64, ny=64, nz=64, q=19, tau=0.8, sigma=0.01, beta=0.7, conservative_omega2=cons)
+        with cg.Domain(cfg) as d:
+            d.init_bubble(radius=12.0, width=4.0)
+            m0 = d.reduce()
+            d.step(100)
+            m100 = d.reduce()
+            d.step(900)
+            m1000 = d.reduce()
+        key = "conservative_omega2" if cons else "reference_mode"
+        out[key] = {"steps_100": [float((m100[k] - m0[k]) / m0[k]) for k in (0, 1)],
+                    "steps_1000": [float((m1000[k] - m0[k]) / m0[k]) for k in (0, 1)]}
+    return out
+
+
+# binary32 VALU issue rate of one MI355X without packed math: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def profile_counters(workload: str, leg: str):
+    """Per-launch counters of the leg's dominant kernel from the committed rocprof summary
+    (profiles/counters_<workload>.json: calibrated HBM bytes per launch and VALU lane-ops per cell
+    update, from SQ_INSTS_VALU x 64 / cells)."""
+    path = os.path.join(ROOT, "profiles", f"counters_{workload}.json")
     try:
         with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+            return json.load(f).get(leg)
     except (OSError, ValueError):
         return None
 
 
-def cpu_baseline(seconds_budget: float = 20.0):
-    """The reference's own update (host build from the reference sources, periodic-y bubble,
-    128^3 interior) timed on this host; falls back to the oracle port if that build is absent."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+def roofline(bytes_per_update: int, cells: int, ms_per_step: float, counters) -> dict:
+    """Both roofs of the step kernel. HBM: algorithmic bytes per update x updates / step time
+    against 8 TB/s. VALU: lane-ops per update (profile) x updates / step time against the binary32
+    VALU issue rate. `bound` names the roof with the larger fraction; the top-level achieved, peak,
+    unit and frac are that roof's (both are kept in `hbm` and `valu`)."""
+    ups = cells / (ms_per_step / 1e3)
+    gbs = bytes_per_update * ups / 1e9
+    hbm = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+           "bytes_per_update": bytes_per_update}
+    out = {"bound": "hbm", **{k: hbm[k] for k in ("achieved", "peak", "unit", "frac")},
+           "traffic": counters.get("hbm_bytes_per_launch") if counters else None, "hbm": hbm, "valu": None}
+    if counters and counters.get("valu_lane_ops_per_update"):
+        lops = counters["valu_lane_ops_per_update"]
+        rate = lops * ups / 1e12
+        valu = {"lane_ops_per_update": lops, "achieved": round(rate, 2), "peak": round(VALU_PEAK_TOPS, 2),
+                "unit": "T lane-ops/s", "frac": round(rate / VALU_PEAK_TOPS, 4), "source": counters.get("source")}
+        out["valu"] = valu
+        if valu["frac"] > hbm["frac"]:
+            out.update(bound="valu", achieved=valu["achieved"], peak=valu["peak"], unit=valu["unit"], frac=valu["frac"])
+    return out
+
+
+def precision_table():
+    """Per-result error of FP16-deviation (and FP32) storage against FP64 runs, per workload
+    (profiles/precision_fp16_vs_fp64.json, written by tests/precision_table.py)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "precision_fp16_vs_fp64.json")) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _cpu_threads() -> int:
+    """Host threads for the CPU baseline: the OMP_NUM_THREADS share this process was given, else
+    every CPU it may run on."""
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if env > 0:
+        return env
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _time_reference(exe: str, mode: str, arg: str, threads: int, budget_s: float, fixed_steps: int = 0):
+    """Seconds per step of the reference's own update (oracle/_ref host build) on `threads` host
+    threads: one probe step (after one untimed step), then as many timed steps as fit `budget_s`."""
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_bubble128_q19")
-    if os.path.exists(exe):
-        probe = subprocess.run([exe, "time_bubble_tanh", "4", "1", "-"], env=env, capture_output=True, text=True,
-                               check=True, timeout=300)
+    k = fixed_steps
+    if not k:
+        probe = subprocess.run([exe, mode, arg, "1", "-"], env=env, capture_output=True, text=True, check=True,
+                               timeout=600)
         sps = float(probe.stdout.split()[1])
-        k = max(2, min(200, int(seconds_budget / max(sps, 1e-3))))
-        run = subprocess.run([exe, "time_bubble_tanh", "4", str(k), "-"], env=env, capture_output=True, text=True,
-                             check=True, timeout=600)
-        sps = float(run.stdout.split()[1])
-        return {"value": 2097152 / sps / 1e6, "unit": "MLUPS", "cores": threads, "kind": "reference",
+        k = max(2, min(1000, int(budget_s / max(sps, 1e-4))))
+    run = subprocess.run([exe, mode, arg, str(k), "-"], env=env, capture_output=True, text=True, check=True,
+                         timeout=900)
+    words = run.stdout.split()
+    return float(words[1]), int(words[3]), k
+
+
+def cpu_baseline(budget_s: float = 4.0):
+    """The reference's own update (compiled for the host from the reference sources, oracle/_ref)
+    timed on this host's cores: the 128^3 D3Q19 drop of the metric (the value), and the other
+    cases of the CPU-baseline table (64^3 static bubble for its full 1000 steps, 128^3 D3Q27 drop,
+    the D3Q27 jet at 128 x 256 x 128, the 128 x 512 x 128 Rayleigh-Taylor domain, and the 128^3
+    drop on one thread), each a bounded sample. Falls back to the oracle port if the reference
+    builds are absent."""
+    threads = _cpu_threads()
+    ref = os.path.join(ROOT, "oracle", "_ref")
+    exe = os.path.join(ref, "ref_bubble128_q19")
+    info = {"cpu_model": _cpu_model(), "host_cpus": os.cpu_count(), "threads": threads}
+    if os.path.exists(exe):
+        sps, cells, k = _time_reference(exe, "time_bubble_tanh", "4", threads, 5.0)
+        table = []
+        cases = [
+            ("static bubble 64^3", 19, "ref_bubble64_q19", "time_bubble_tanh", "4", threads, 1000),
+            ("drop 128^3", 27, "ref_bubble128_q27", "time_bubble_tanh", "4", threads, 0),
+            ("round jet 128 x 256 x 128 (reference convective outlet)", 27, "ref_jet256_q27", "time_jet", "0", threads, 0),
+            ("Rayleigh-Taylor 128 x 512 x 128 (no body force in the reference)", 19, "ref_rti512_q19", "time_rti", "0",
+             threads, 0),
+            ("drop 128^3, one thread", 19, "ref_bubble128_q19", "time_bubble_tanh", "4", 1, 0),
+        ]
+        for label, q, binary, mode, arg, th, fixed in cases:
+            path = os.path.join(ref, binary)
+            if not os.path.exists(path):
+                continue
+            t, n, kk = _time_reference(path, mode, arg, th, budget_s, fixed)
+            table.append({"case": label, "q": q, "threads": th, "timed_steps": kk, "cells": n,
+                          "value": round(n / t / 1e6, 2), "unit": "MLUPS"})
+            print(f"bench: CPU baseline {label} D3Q{q}: {n / t / 1e6:.1f} MLUPS on {th} threads", file=sys.stderr,
+                  flush=True)
+        return {"value": round(cells / sps / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "reference",
                 "sample": f"{k} timed steps (after 1 warm-up) of the same D3Q19 128^3 drop; reference lbm.cuh RCS "
-                          f"built for the host, y wrapped through its own wrapy"}
+                          f"built for the host (g++ -O2, OpenMP over z), y wrapped through its own wrapy",
+                **info, "table": table}
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import cgref
     p = cgref.Params(nx=128, ny=128, nz=128, q=19, tau=0.8, sigma=0.01, beta=0.7)
@@ -241,13 +358,13 @@ def cpu_baseline(seconds_budget: float = 20.0):
     cgref.step(p, A, B, 1)
     t0 = time.perf_counter()
     k = 0
-    while time.perf_counter() - t0 < seconds_budget or k < 2:
+    while time.perf_counter() - t0 < budget_s or k < 2:
         cgref.step(p, B, A, k + 2)
         A, B = B, A
         k += 1
     dt = (time.perf_counter() - t0) / k
     return {"value": 2097152 / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"{k} steps of the same D3Q19 128^3 drop with the oracle"}
+            "sample": f"{k} steps of the same D3Q19 128^3 drop with the oracle", **info}
 
 
 def main() -> int:
@@ -309,7 +426,12 @@ def main() -> int:
     if args.calibrate_copy:
         calibrate_copy(cg, args.calibrate_copy)
     workload = f"bubble{args.n}_d3q19"
-    traffic = traffic_from_profiles(workload)
+    prof = f"bubble{args.n}"
+    cells = args.n ** 3
+    roof19 = roofline(BYTES_PER_UPDATE_F32, cells, r19["ms_per_step"], profile_counters(prof, "d3q19"))
+    roof27 = roofline(BYTES_PER_UPDATE_F32, cells, r27["ms_per_step"], profile_counters(prof, "d3q27"))
+    roof16 = roofline(BYTES_PER_UPDATE_F16, cells, r16["ms_per_step"], profile_counters(prof, "d3q19_f16"))
+    prec = precision_table()
     cpu = None
     if not args.no_cpu:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
@@ -329,18 +451,21 @@ def main() -> int:
         "data": "synthetic: periodic 128^3 domain, diffuse drop R=24 (tanh, w=4), rho=1, u=Pi=0 at t=0",
         "config": {"workload": workload, "grid": [args.n] * 3, "q": 19, "tau": 0.8, "sigma": 0.01, "beta": 0.7,
                    "drop_radius": 24.0 * args.n / 128.0, "storage": "f32", "kernel": args.kernel,
-                   "tile_d3q19": r19["tile"], "tile_d3q27": r27["tile"]},
-        "roofline": {"bound": "hbm", "achieved": round(r19["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r19["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic},
+                   "tile_d3q19": r19["tile"], "tile_d3q27": r27["tile"],
+                   "timing": "HIP events on the domain stream around exactly `steps` steps; with planned chunks "
+                             "(chunk planes 0) a re-plan (recording step plus planner, ~50 us) falls every 64 "
+                             "steps, so fewer timed steps than that include at most one"},
+        "hbm_frac": roof19["hbm"]["frac"],
+        "roofline": roof19,
         "cpu_baseline": cpu,
         "d3q27": {"value": round(r27["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(r27["ms_per_step"], 5),
-                  "roofline": {"bound": "hbm", "achieved": round(r27["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
-                               "unit": "GB/s", "frac": round(r27["achieved_gbs"] / HBM_PEAK_GBS, 4)}},
+                  "roofline": roof27},
         "d3q19_f16_storage": {"value": round(r16["mlups"], 1), "unit": "MLUPS", "ms_per_step": round(r16["ms_per_step"], 5),
-                              "bytes_per_update": BYTES_PER_UPDATE_F16, "tile": r16["tile"],
-                              "roofline": {"bound": "hbm", "achieved": round(r16["achieved_gbs"], 1),
-                                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                           "frac": round(r16["achieved_gbs"] / HBM_PEAK_GBS, 4)}},
+                              "bytes_per_update": BYTES_PER_UPDATE_F16, "tile": r16["tile"], "roofline": roof16,
+                              "error_vs_fp64": prec.get(f"drop{args.n}_d3q19"),
+                              "error_note": "relative RMS per field against an FP64 run of the same drop "
+                                            "(profiles/precision_fp16_vs_fp64.json; the FP16 storage model there is "
+                                            "this path's bit for bit)"},
         "mass_drift_rel": {"d3q19": r19["mass_drift_rel"], "d3q27": r27["mass_drift_rel"],
                            "d3q19_f16_storage": r16["mass_drift_rel"]},
         "batched": batched,

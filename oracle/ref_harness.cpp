@@ -49,6 +49,7 @@ inline label_t ref_force_y_periodic(const int y) noexcept { return wrapy(static_
 #include "rti/rtiIn.cuh"
 #include "io/vtiWriter.cuh"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -206,20 +207,42 @@ int main(int argc, char **argv) {
     }
     const std::string prefix = argv[4];
 
-    if (mode == "time_bubble_tanh") {
-        // CPU-baseline timing: diffuse drop, one untimed step, then `steps` timed steps;
-        // prints seconds per step (argv[3] = number of timed steps, argv[4] ignored).
-        if (!g_periodic_y) { std::fprintf(stderr, "needs the periodic-y build\n"); return 1; }
+    if (mode.rfind("time_", 0) == 0) {
+        // CPU-baseline timing: the initial state of the case (time_bubble_tanh: diffuse drop of
+        // width argv[2], periodic-y build; time_jet: launch_InitJet + main.cu:26; time_rti: the
+        // rtiIn.cuh profile, walls held), one untimed step, then argv[3] timed steps of the
+        // reference's step (RCS over every interior cell, plus the jet boundary kernel for the
+        // jet); prints seconds per step.
+        const std::string what = mode.substr(5);
+        const bool ok = what == "bubble_tanh" ? g_periodic_y : (what == "jet" || what == "rti") && !g_periodic_y;
+        if (!ok) { std::fprintf(stderr, "%s needs the %s build\n", mode.c_str(), g_periodic_y ? "jet" : "periodic-y"); return 1; }
         HostState *A = new HostState, *B = new HostState;
-        init_bubble_tanh(*A, arg);
-        rcs_all(*A, *B);
-        std::swap(A, B);
+        if (what == "bubble_tanh") {
+            init_bubble_tanh(*A, arg);
+        } else if (what == "jet") {
+            for (label_t z = 0; z < NZ; ++z)
+                for (label_t y = 0; y < NY; ++y)
+                    for (label_t x = 0; x < NX; ++x) init_jet_bulk(A->view, x, y, z);
+            for (label_t z = 0; z < NZ; ++z)
+                for (label_t x = 0; x < NX; ++x)
+                    if (!inlet_outlet_interior(x, z)) init_jet_inlet(A->view, x, z);
+            jet_bc(*A, *A);
+        } else {
+            for (label_t z = 0; z < NZ; ++z)
+                for (label_t y = 0; y < NY; ++y)
+                    for (label_t x = 0; x < NX; ++x) init_rti_domain(A->view, x, y, z);
+            // the wall planes, which the step never writes, hold in both buffers
+            for (int f = 0; f < 11; ++f) std::copy(A->buf[f].begin(), A->buf[f].end(), B->buf[f].begin());
+        }
+        auto one = [&]() {
+            rcs_all(*A, *B);
+            if (what == "jet") jet_bc(*B, *A);
+            std::swap(A, B);
+        };
+        one();
         const int steps = dumps.empty() ? 1 : dumps[0];
         const auto t0 = std::chrono::steady_clock::now();
-        for (int k = 0; k < steps; ++k) {
-            rcs_all(*A, *B);
-            std::swap(A, B);
-        }
+        for (int k = 0; k < steps; ++k) one();
         const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         std::printf("seconds_per_step %.9f cells %llu\n", dt / steps,
                     static_cast<unsigned long long>(NX - 2) * (NY - 2) * (NZ - 2));

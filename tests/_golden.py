@@ -13,9 +13,17 @@ GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def names():
-    """State fixtures (the VTI fixtures, vti_*.npz, have their own format and tests)."""
+    """State fixtures of the small cases (the VTI fixtures, vti_*.npz, and the full-size ones,
+    size_*.npz, have their own formats and tests)."""
     return sorted(n for n in (os.path.splitext(os.path.basename(p))[0]
-                              for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz"))) if not n.startswith("vti_"))
+                              for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz")))
+                  if not n.startswith(("vti_", "size_")))
+
+
+def size_names():
+    """Full-size fixtures (tests/golden/make_size_golden.py): per-field sha256 of the benched and
+    configured workloads at their real sizes."""
+    return sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(GOLDEN_DIR, "size_*.npz")))
 
 
 def load(name: str):

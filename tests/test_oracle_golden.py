@@ -40,3 +40,26 @@ def test_oracle_rti_init_matches_reference(oracle):
     ref = z["full_0"]
     np.testing.assert_allclose(A[:2], ref[:2], rtol=0, atol=1e-6)
     np.testing.assert_array_equal(A[2:], ref[2:])
+
+
+def test_full_size_fixtures_pin_the_oracle():
+    """The full-size fixtures written from the reference's own update (tests/golden/size_*.npz with
+    source "reference": the 128^3 bench drop, D3Q19 and D3Q27, and the 64^3 static bubble to 1000
+    steps) were checked against the oracle when they were written: same sha256 for every field at
+    every snapshot. The oracle-written ones (jet with inlet noise and zero-gradient outflow, forced
+    Rayleigh-Taylor case) name their parameters. All name the same reference sources as the small
+    fixtures."""
+    small = _golden.load("bubble_q19")[1]["reference_sha256"]
+    names = _golden.size_names()
+    assert {n for n in names if n.startswith("size_bubble")} == {"size_bubble128_q19", "size_bubble128_q27",
+                                                                  "size_bubble64_q19"}
+    for name in names:
+        z, meta = _golden.load(name)
+        assert len(z["sha_0"]) == 11
+        for k in meta["steps"]:
+            assert len(z[f"sha_{k}"]) == 11 and z[f"val_{k}"].shape == (11, 256)
+        if meta["source"] == "reference":
+            assert meta["oracle_matches"] is True
+            assert meta["reference_sha256"] == small
+        else:
+            assert meta["source"] == "oracle" and meta["steps"] == [100]
