@@ -60,6 +60,7 @@ def test_full_size_fixtures_pin_the_oracle():
             assert len(z[f"sha_{k}"]) == 11 and z[f"val_{k}"].shape == (11, 256)
         if meta["source"] == "reference":
             assert meta["oracle_matches"] is True
-            assert meta["reference_sha256"] == small
+            ref = meta["reference_sha256"]
+            assert all(ref[f] == h for f, h in small.items() if f in ref) and set(small) <= set(ref)
         else:
             assert meta["source"] == "oracle" and meta["steps"] == [100]
