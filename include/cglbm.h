@@ -130,6 +130,20 @@ int cglbm_plan_reset(cglbm_plan *plan);
 int cglbm_launch_rcs_planned(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
                              const cglbm_layout *layout, int64_t step, cglbm_plan *plan, void *stream);
 
+/* cglbm_launch_rcs_planned without a plan object and without any device allocation. No
+ * counterpart in the reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The plan lives in a file-scope device
+ * array of 384 KiB per GPU (held until the process ends), enough for a single-round plan of 16-row
+ * column tiles while ceil(nx/32) ceil(ny/16) nz <= ~48,000 (cubes up to 256^3 on 256 CUs); larger domains
+ * get CGLBM_EINVAL with the bytes needed. Domains whose plan would need several rounds of
+ * work-groups run as cglbm_launch_rcs (no scratch). The plan follows the last call's extents and
+ * stencil: a call with another shape starts a fresh plan (results are bit for bit those of
+ * cglbm_launch_rcs either way). Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (two host threads, or two streams running at once) race; a process stepping
+ * several domains on one GPU should keep a cglbm_plan per domain instead. */
+int cglbm_launch_rcs_planned_noalloc(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
+                                     const cglbm_layout *layout, int64_t step, void *stream);
+
 /* Boundary planes only: JetBoundaryConditions(B, Aold) (kernels.cu:102-114) or the wall copies.
  * B == Aold is allowed (main.cu:26). */
 int cglbm_launch_ybc(const cglbm_moments *B, const cglbm_moments *Aold, const cglbm_config *cfg,

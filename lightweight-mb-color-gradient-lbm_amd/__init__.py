@@ -38,6 +38,7 @@ EXPORTS = (
     "cglbm_domain_step_count", "cglbm_domain_stream", "cglbm_domain_plan",
     "cglbm_plan_create", "cglbm_plan_destroy", "cglbm_plan_reset", "cglbm_launch_rcs_planned",
     "cglbm_launch_init_bubble", "cglbm_launch_init_jet", "cglbm_launch_init_rti",
+    "cglbm_launch_rcs_planned_noalloc",
     "cglbm_init_bubble", "cglbm_init_jet", "cglbm_init_rti", "cglbm_step", "cglbm_time_steps", "cglbm_sync",
     "cglbm_copy_out", "cglbm_copy_in", "cglbm_reduce", "cglbm_write_vti", "cglbm_write_vti_host",
     "cglbm_batch_create", "cglbm_batch_destroy", "cglbm_batch_add", "cglbm_batch_remove", "cglbm_batch_size",
@@ -184,6 +185,7 @@ def lib():
         "cglbm_plan_destroy": ([ctypes.c_void_p], ctypes.c_int),
         "cglbm_plan_reset": ([ctypes.c_void_p], ctypes.c_int),
         "cglbm_launch_rcs_planned": ([M, M, C, LP, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+        "cglbm_launch_rcs_planned_noalloc": ([M, M, C, LP, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
         "cglbm_domain_create": ([C, ctypes.c_int32, ctypes.POINTER(D)], ctypes.c_int),
         "cglbm_domain_destroy": ([D], ctypes.c_int),
         "cglbm_domain_moments": ([D, ctypes.c_int32, M], ctypes.c_int),
@@ -298,6 +300,16 @@ def launch_rcs_planned(A: Moments, B: Moments, cfg: Config, plan: Plan, layout: 
     L = layout if layout is not None else compact_layout(cfg)
     _check(lib().cglbm_launch_rcs_planned(ctypes.byref(A), ctypes.byref(B), ctypes.byref(cc), ctypes.byref(L), step,
                                           plan._h, stream), "cglbm_launch_rcs_planned")
+
+
+def launch_rcs_planned_noalloc(A: Moments, B: Moments, cfg: Config, layout: Layout | None = None, step: int = 1,
+                               stream: int | None = None):
+    """``cglbm_launch_rcs_planned`` with the plan in the library's per-GPU scratch array (no device
+    allocation); calls that overlap on one GPU race."""
+    cc = cfg.c()
+    L = layout if layout is not None else compact_layout(cfg)
+    _check(lib().cglbm_launch_rcs_planned_noalloc(ctypes.byref(A), ctypes.byref(B), ctypes.byref(cc), ctypes.byref(L),
+                                                  step, stream), "cglbm_launch_rcs_planned_noalloc")
 
 
 def launch_ybc(B: Moments, Aold: Moments, cfg: Config, layout: Layout | None = None, step: int = 0,

@@ -427,6 +427,52 @@ struct cglbm_plan {
     cglbm::MarchPlanState st;
 };
 
+// ---- Scratch of the no-allocation planned launch ---------------------------------------------------
+// cglbm_launch_rcs_planned_noalloc keeps its chunk plan (work items, their measured durations and
+// the interface map) in this file-scope device array instead of a cglbm_plan: each GPU the library
+// runs on holds its own copy until the process ends. Host code never passes the array's name to a
+// kernel or a copy: it takes the device address once per device with hipGetSymbolAddress and uses
+// that. Nothing in it needs to start at zero: after a change of domain shape the first call records
+// a complete interface map (every column tile and plane) before any plan is read.
+constexpr size_t kNoAllocScratchBytes = 384 * 1024;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+struct NoAllocSlot {
+    bool ready = false;
+    unsigned char *ptr = nullptr;  // this device's copy of `scratch`
+    cglbm_config key{};            // extents and stencil the current plan was made for
+    MarchPlanState st;             // host side of the plan, its arrays inside `ptr`
+};
+std::mutex g_noalloc_mu;
+NoAllocSlot g_noalloc[kMaxDevices];
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// The slot of the current device, set up on its first use (symbol address, CU count).
+int noalloc_slot(NoAllocSlot **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(CGLBM_EINVAL, "device ordinal %d beyond %d", dev, kMaxDevices);
+    NoAllocSlot &sl = g_noalloc[dev];
+    if (!sl.ready) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        int ncu = 0;
+        e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
+        sl.ptr = static_cast<unsigned char *>(p);
+        sl.st.ncu = ncu;
+        sl.ready = true;
+    }
+    *out = &sl;
+    return CGLBM_OK;
+}
+}  // namespace
+
 // ------------------------------------------------------------------------------------------------
 extern "C" {
 
@@ -511,6 +557,43 @@ int cglbm_launch_rcs_planned(const cglbm_moments *A, const cglbm_moments *B, con
         return launch_step(A, B, cfg, layout, step, CGLBM_KERNEL_AUTO, nullptr, s);
     const MarchShape m{16, 0, cfg->q == 27 ? 1 : 2};
     return launch_step(A, B, cfg, layout, step, CGLBM_KERNEL_AUTO, &m, s, 0, 0, &plan->st);
+}
+
+// cglbm_launch_rcs_planned with the plan in the per-device `scratch` array: no allocation, no plan
+// object. Calls that overlap on one GPU share that array and race.
+int cglbm_launch_rcs_planned_noalloc(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
+                                     const cglbm_layout *layout, int64_t step, void *stream) {
+    if (int rc = validate_config(cfg)) return rc;
+    if (int rc = check_layout(cfg, layout)) return rc;
+    if (int rc = check_moments(A, "A")) return rc;
+    if (int rc = check_moments(B, "B")) return rc;
+    if (int rc = check_disjoint(A, B, cfg, layout)) return rc;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    NoAllocSlot *sl = nullptr;
+    if (int rc = noalloc_slot(&sl)) return rc;
+    const int ncu = sl->st.ncu;
+    // as cglbm_launch_rcs_planned: plans of one round of work-groups only
+    if (plan_items(*cfg, 16, ncu) > ncu) return launch_step(A, B, cfg, layout, step, CGLBM_KERNEL_AUTO, nullptr, s);
+    const size_t items_b = align256(sizeof(int4) * static_cast<size_t>(ncu));
+    const size_t itime_b = align256(sizeof(uint32_t) * static_cast<size_t>(ncu));
+    const size_t wmap_b = align256(static_cast<size_t>(ceil_div(cfg->nx, kTX)) * ceil_div(cfg->ny, 16) *
+                                   static_cast<size_t>(cfg->nz) * 8);
+    if (items_b + itime_b + wmap_b > kNoAllocScratchBytes)
+        return fail(CGLBM_EINVAL, "the no-allocation planned launch needs %zu bytes of scratch for a %d x %d x %d domain; "
+                    "it has %zu (use cglbm_launch_rcs_planned with a cglbm_plan)", items_b + itime_b + wmap_b, cfg->nx,
+                    cfg->ny, cfg->nz, kNoAllocScratchBytes);
+    const cglbm_config &k = sl->key;
+    if (k.nx != cfg->nx || k.ny != cfg->ny || k.nz != cfg->nz || k.q != cfg->q || k.ybc != cfg->ybc) {
+        sl->key = *cfg;
+        sl->st.ty = 0;  // another domain shape: record a fresh interface map first
+    }
+    sl->st.items = reinterpret_cast<int4 *>(sl->ptr);
+    sl->st.itime = reinterpret_cast<uint32_t *>(sl->ptr + items_b);
+    sl->st.wmap = sl->ptr + items_b + itime_b;
+    sl->st.cap = ncu;
+    const MarchShape m{16, 0, cfg->q == 27 ? 1 : 2};
+    return launch_step(A, B, cfg, layout, step, CGLBM_KERNEL_AUTO, &m, s, 0, 0, &sl->st);
 }
 
 // launch_JetBoundaryConditions (launch.cuh:78-83).
