@@ -238,6 +238,20 @@ __device__ __forceinline__ float div_shared(float a, float b, float y) {
     return q;
 }
 
+// a / b given y = RN(1 / b), without div_shared's zero and tiny-numerator corrections: used only
+// for the Omega2 direction term c2 = (F.c)^2 / |F|^2 of a source with |F| > 1e-4, where it gives
+// the same om as the correctly rounded quotient. The numerator (F.c)^2 is never -0, and a zero
+// numerator gives +0 here as in IEEE division. The result differs from RN(a / b) only when the
+// remainder underflows, which needs a < 2^-100; the quotient is then below 2^-100 / 1e-8 (|F|^2 >
+// 1e-8), and this form returns a value within a few ulps of it. Either value, times the weight,
+// is below 1e-23, less than half an ulp of every B_i (at least 1/36, ulp 1.9e-9), so
+// w c2 - B_i rounds to -B_i exactly and om is the same bits.
+__device__ __forceinline__ float div_markstein(float a, float b, float y) {
+    float q = a * y;
+    const float r = fmaf(-q, b, a);
+    return fmaf(r, y, q);
+}
+
 __device__ __forceinline__ Iface iface_begin(const Mom &s) {
     Iface f;
     const float inv = 1.0f / (s.rr + s.rb);
@@ -333,10 +347,7 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
         } else {
             g = feq0 + c.oms * gneq;
         }
-        if constexpr (IFACE) {
-            const float om = fi.ha * (-bcoef<Q>(0));
-            if (fi.omega) g += om;
-        }
+        if constexpr (IFACE) g += fi.omega ? fi.ha * (-bcoef<Q>(0)) : -0.0f;
         sink(std::integral_constant<int, 0>{}, aR * g, aB * g);
     }
     unroll<0, (Q - 1) / 2>([&](auto K) {
@@ -376,11 +387,10 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
                 const float Fc = cdot<I>(fi.Fx, fi.Fy, fi.Fz);
                 const float c2q = (Fc * Fc) / fi.af2;
                 const float c2 = fi.small ? 0.0f : c2q;
-                const float om = fi.ha * (w * c2 - bcoef<Q>(I));
-                if (fi.omega) {
-                    gi += om;
-                    go += om;
-                }
+                // a skipped Omega2 adds -0, which leaves every value as it is
+                const float om = fi.omega ? fi.ha * (w * c2 - bcoef<Q>(I)) : -0.0f;
+                gi += om;
+                go += om;
                 const float cq = Fc * invcnorm(I) / fi.af;
                 const float cphi = fi.small ? 0.0f : cq;
                 const float D = fi.coeff * cphi * wr;
@@ -388,20 +398,23 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
                 sink(std::integral_constant<int, O>{}, aR * go - D, aB * go + D);
             } else if constexpr (IFACE) {
                 const float Fc = cdot<I>(fi.Fx, fi.Fy, fi.Fz);
-                const float c2q = div_shared(Fc * Fc, fi.af2, fi.rinv_af2);
+                const float c2q = div_markstein(Fc * Fc, fi.af2, fi.rinv_af2);
                 const float c2 = fi.small ? 0.0f : c2q;
-                const float om = fi.ha * (w * c2 - bcoef<Q>(I));
-                if (fi.omega) {
-                    gi += om;
-                    go += om;
-                }
+                // a skipped Omega2 adds -0, which leaves every value as it is
+                const float om = fi.omega ? fi.ha * (w * c2 - bcoef<Q>(I)) : -0.0f;
+                gi += om;
+                go += om;
                 const float cq = div_shared(Fc * invcnorm(I), fi.af, fi.rinv_af);
                 const float cphi = fi.small ? 0.0f : cq;
                 const float D = fi.coeff * cphi * wr;
                 const float ri = aR * gi, bi = aB * gi, ro = aR * go, bo = aB * go;
-                // a non-interface lane running with an interface wave adds no recolouring term
-                sink(std::integral_constant<int, I>{}, fi.on ? ri + D : ri, fi.on ? bi - D : bi);
-                sink(std::integral_constant<int, O>{}, fi.on ? ro - D : ro, fi.on ? bo + D : bo);
+                // A non-interface lane running with an interface wave adds no recolouring term: it
+                // adds -0 where the term is added and subtracts +0 where it is subtracted, both of
+                // which leave every value (signed zeros included) as it is.
+                const float Dp = fi.on ? D : -0.0f;
+                const float Dm = fi.on ? D : 0.0f;
+                sink(std::integral_constant<int, I>{}, ri + Dp, bi - Dm);
+                sink(std::integral_constant<int, O>{}, ro - Dm, bo + Dp);
             } else {
                 sink(std::integral_constant<int, I>{}, aR * gi, aB * gi);
                 sink(std::integral_constant<int, O>{}, aR * go, aB * go);

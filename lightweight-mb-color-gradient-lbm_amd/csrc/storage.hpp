@@ -48,8 +48,12 @@ __host__ __device__ constexpr int field_bytes(int store, int f) { return (store 
 __device__ __forceinline__ uint16_t f16_encode(int f, float v) {
     return __builtin_bit_cast(uint16_t, static_cast<_Float16>(v * f16_scale(f)));
 }
+// The product of the exact conversion and a power of two is exact (binary16 values times 2^-10 or
+// 2^-12 stay normal binary32), so one fused multiply-add with -0 (which keeps the product's sign of
+// zero) gives the same bits as the conversion followed by the multiplication, in one mixed-precision
+// instruction.
 __device__ __forceinline__ float f16_decode(int f, uint16_t h) {
-    return static_cast<float>(__builtin_bit_cast(_Float16, h)) * f16_inv_scale(f);
+    return __builtin_fmaf(static_cast<float>(__builtin_bit_cast(_Float16, h)), f16_inv_scale(f), -0.0f);
 }
 
 }  // namespace cglbm

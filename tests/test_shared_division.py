@@ -62,3 +62,58 @@ def test_shared_reciprocal_division_is_correctly_rounded():
         n, bad = (int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split())
     assert n > 45_000_000
     assert bad == 0
+
+
+SRC_OMEGA = r"""
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+static float fb(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static uint32_t bf(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static uint64_t s = 0x9E3779B97F4A7C15ull;
+static uint64_t nxt(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return s; }
+static float markstein(float a, float b, float y) { float q = a * y; float r = fmaf(-q, b, a); return fmaf(r, y, q); }
+/* D3Q19 weights of the moving directions (B_i = w_i for i >= 1), as binary32 */
+static const float W[2] = {(float)(1.0 / 18.0), (float)(1.0 / 36.0)};
+int main(void) {
+    long n = 0, bad = 0;
+    volatile float one = 1.0f;
+    for (long i = 0; i < 4000000L; ++i) {
+        /* |F| > 1e-4: af in [1e-4, 8], a numerator (F.c)^2 anywhere from 0 and the smallest
+           subnormal up to 3 af^2 (F.c can cancel to anything) */
+        float af = 1e-4f * (1.0f + (float)((double)(nxt() >> 11) / 9007199254740992.0) * 79999.0f);
+        if (!(af > 1e-4f)) continue;
+        float af2 = af * af, y = one / af2;
+        float a;
+        int kind = (int)(nxt() % 4);
+        if (kind == 0) a = fb((uint32_t)(nxt() % 0x0D000000u));              /* 0 .. ~2^-101, subnormals included */
+        else if (kind == 1) a = fb((uint32_t)(nxt() & 0x007FFFFFu) | ((uint32_t)(nxt() % 40) << 23));
+        else { float fc = af * (float)((double)(nxt() >> 11) / 9007199254740992.0 * 1.7320508); a = fc * fc; }
+        float ha = 0.5f * 0.0212f * af;
+        for (int k = 0; k < 2; ++k) {
+            float c2i = a / af2, c2m = markstein(a, af2, y);
+            float omi = ha * (W[k] * c2i - W[k]), omm = ha * (W[k] * c2m - W[k]);
+            ++n; bad += bf(omi) != bf(omm);
+        }
+    }
+    printf("%ld %ld\n", n, bad);
+    return 0;
+}
+"""
+
+
+def test_markstein_quotient_gives_the_same_omega2():
+    """div_markstein (march_core.hpp) skips div_shared's tiny-numerator and zero corrections for the
+    Omega2 term c2 = (F.c)^2 / |F|^2 of sources with |F| > 1e-4: the resulting
+    om = (A|F|/2)(w c2 - B) must be the IEEE-division result bit for bit, including numerators
+    that are zero, subnormal or far below 2^-100 (F.c cancelling)."""
+    with tempfile.TemporaryDirectory() as tmp:
+        c = os.path.join(tmp, "om.c")
+        with open(c, "w") as f:
+            f.write(SRC_OMEGA)
+        exe = os.path.join(tmp, "om")
+        subprocess.run(["gcc", "-O2", "-mfma", "-ffp-contract=off", c, "-o", exe, "-lm"], check=True)
+        n, bad = (int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split())
+    assert n > 7_000_000
+    assert bad == 0
