@@ -70,6 +70,11 @@ def mass(p, a):
     return s
 
 
+def drift(m, m0):
+    """Relative per-phase mass change; None for a phase that starts empty (the jet's injected one)."""
+    return [float(f"{(a - b) / b:.3g}") if b > 0 else None for a, b in zip(m, m0)]
+
+
 def main() -> int:
     import cgref
 
@@ -98,8 +103,8 @@ def main() -> int:
             a = fn(p, A, w["steps"])
             e = rel_rms(p, a, ref)
             res[label] = {f: float(f"{v:.3g}") for f, v in zip(FIELDS, e)}
-            res[label]["mass_drift"] = [float(f"{v:.3g}") for v in ((mass(p, a) - m0) / np.where(m0 > 0, m0, 1))]
-        res["fp64_mass_drift"] = [float(f"{v:.3g}") for v in ((mass(p, ref) - m0) / np.where(m0 > 0, m0, 1))]
+            res[label]["mass_drift"] = drift(mass(p, a), m0)
+        res["fp64_mass_drift"] = drift(mass(p, ref), m0)
         out[name] = res
         print(f"{name}: {time.time() - t0:.0f} s; f16 {res['f16']}", file=sys.stderr, flush=True)
         with open(OUT, "w") as f:
