@@ -469,7 +469,13 @@ def main() -> int:
         "mass_drift_rel": {"d3q19": r19["mass_drift_rel"], "d3q27": r27["mass_drift_rel"],
                            "d3q19_f16_storage": r16["mass_drift_rel"]},
         "batched": batched,
-        "serving": serving,
+        "serving": None if serving is None else {
+            **serving,
+            "fp16_error_vs_fp64_by_kind": {k: prec.get(n, {}).get("f16") for k, n in (
+                ("bubble", "serving_bubble_d3q19_48"), ("jet", "serving_jet_d3q27_64x128x64"),
+                ("rti", "serving_rti_d3q19_64x256x64"))},
+            "fp16_error_note": "relative RMS per field after 500 steps against FP64, per case kind at a mid-range draw "
+                               "(profiles/precision_fp16_vs_fp64.json)"},
         "other_configs": configs,
     }
     print(json.dumps(line))

@@ -74,8 +74,9 @@ static uint32_t bf(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 static uint64_t s = 0x9E3779B97F4A7C15ull;
 static uint64_t nxt(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return s; }
 static float markstein(float a, float b, float y) { float q = a * y; float r = fmaf(-q, b, a); return fmaf(r, y, q); }
-/* D3Q19 weights of the moving directions (B_i = w_i for i >= 1), as binary32 */
-static const float W[2] = {(float)(1.0 / 18.0), (float)(1.0 / 36.0)};
+/* weights of the moving directions (B_i = w_i for i >= 1), D3Q19 and D3Q27, as binary32 */
+static const float W[5] = {(float)(1.0 / 18.0), (float)(1.0 / 36.0), (float)(2.0 / 27.0), (float)(1.0 / 54.0),
+                           (float)(1.0 / 216.0)};
 int main(void) {
     long n = 0, bad = 0;
     volatile float one = 1.0f;
@@ -91,7 +92,7 @@ int main(void) {
         else if (kind == 1) a = fb((uint32_t)(nxt() & 0x007FFFFFu) | ((uint32_t)(nxt() % 40) << 23));
         else { float fc = af * (float)((double)(nxt() >> 11) / 9007199254740992.0 * 1.7320508); a = fc * fc; }
         float ha = 0.5f * 0.0212f * af;
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < 5; ++k) {
             float c2i = a / af2, c2m = markstein(a, af2, y);
             float omi = ha * (W[k] * c2i - W[k]), omm = ha * (W[k] * c2m - W[k]);
             ++n; bad += bf(omi) != bf(omm);
@@ -115,5 +116,5 @@ def test_markstein_quotient_gives_the_same_omega2():
         exe = os.path.join(tmp, "om")
         subprocess.run(["gcc", "-O2", "-mfma", "-ffp-contract=off", c, "-o", exe, "-lm"], check=True)
         n, bad = (int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split())
-    assert n > 7_000_000
+    assert n > 19_000_000
     assert bad == 0
