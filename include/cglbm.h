@@ -187,9 +187,10 @@ int cglbm_domain_set_kernel(cglbm_domain *dom, int32_t kernel);
  * cost, one work-group per CU (results are bitwise the same for every shape). ty = 0 returns to
  * automatic selection: the autotuned record, else planned 16-row chunks (when they average at
  * least 2 planes and the planner's LDS holds a byte per (column tile, plane): cubes from 64^3
- * to 256^3), checked once per state against the size-aware uniform shape (three steps of each
- * timed with events on the domain's stream, read without blocking; the faster is kept), else
- * the size-aware uniform shape. */
+ * to 256^3), else the size-aware uniform shape; either races the other default candidates
+ * (the size-aware uniform shape, and 4-row tiles with 32-plane chunks for D3Q19) once the state is
+ * set and every 2048 steps after: three steps of each timed with events on the domain's stream,
+ * the fastest kept (cglbm_step then waits for those few steps once per race). */
 int cglbm_domain_set_tile(cglbm_domain *dom, int32_t ty, int32_t L, int32_t nbuf);
 /* Autotuner: times each candidate tile shape (planned chunks, L = 0, among them) for
  * steps_per_candidate updates from the current state (the state does not advance), keeps the

@@ -96,7 +96,7 @@ int apply_ybc_views(cglbm_domain *d, const cglbm_moments &m) {
 int upload_init(cglbm_domain *d, const std::vector<float> &host, bool bc) {
     const int64_t n = stored_cells(d->cfg);
     d->plan.ty = 0;  // a new state: the next planned step records its interface map afresh
-    d->plan.trial = kTrialNone;  // and planned against uniform chunks is timed again
+    d->plan.trial = kTrialNone;  // and the default shapes race again
     Staging st;
     const cglbm_moments *m = &d->view[d->cur];
     if (d->storage == CGLBM_STORE_F16) {

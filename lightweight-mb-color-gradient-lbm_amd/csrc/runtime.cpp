@@ -239,8 +239,8 @@ int plan_items_for(const cglbm_config &c, int ty, int ncu) {
 // slower than uniform chunks (384^3: 19.6 against 28.4 G updates/s), and at 320^3 (one round,
 // with the planner given more LDS) planning gained nothing. A single-round domain with two chunks
 // per column whose heavy columns are heavy along their whole length (the 128 x 512 x 128
-// Rayleigh-Taylor layer) is planned 19-23% slower than uniform chunks: launch_default times both
-// once and keeps the faster, and the autotuner, which times both, picks uniform chunks there.
+// Rayleigh-Taylor layer) is planned 19-23% slower than uniform chunks: launch_default races the
+// candidates and keeps the fastest, and the autotuner, which times both, picks uniform chunks there.
 MarchShape untuned_shape(const cglbm_config &c, const MarchPlanState *plan, const MarchShape &dflt) {
     if (!plan || plan->ncu < 1) return dflt;
     // a shape forced through the environment (march_default_shape) wins over planning
@@ -289,60 +289,79 @@ hipError_t launch_march_planned(MarchArgs ma, const cglbm_config &c, MarchShape 
     }
     return hipSuccess;
 }
-// The step of a domain left to the default shape (no tile set, no autotuned record): planned chunks
-// where untuned_shape picks them, checked once against the uniform default by timing both on the
-// domain's stream (MarchPlanState::trial). Planned chunks lose where the heavy column tiles are
-// heavy along their whole length and the single round of work-groups cannot even them out (a flat
-// Rayleigh-Taylor layer across x and z); the hardware's own dispatch of many uniform work-groups
-// balances such a case better. The planned steps' share of the re-plans (a recording step and
-// the planner every `period` steps, about 2% of a step) is charged to them.
+// The step of a domain left to the default shape (no tile set, no autotuned record). A short race
+// (MarchPlanState::trial) picks among planned chunks where untuned_shape plans them, the size-aware
+// uniform default, and, for D3Q19, 4-row tiles with 32-plane chunks: each runs kTrialSteps steps
+// timed with events on the domain's stream, the host waits for the last of them (a few
+// milliseconds, once per race), the fastest is kept, and the race runs again every kTrialPeriod
+// steps. Planned chunks lose where the heavy column tiles are heavy along their whole
+// length and one round of work-groups cannot even them out (a flat Rayleigh-Taylor layer across x
+// and z); states too large for the 256 MB last-level cache run faster with the smaller 4-row
+// work-groups, three of which share a CU (the 128 x 512 x 128 Rayleigh-Taylor case in FP32). The
+// planned candidate's share of its re-plans (a recording step and the planner every `period`
+// steps, about 2% of a step) is charged to it. Results are the same bits for every candidate.
 hipError_t launch_default(const MarchArgs &ma, const cglbm_config &c, MarchPlanState *plan, const MarchShape &dflt,
                           hipStream_t s) {
-    const MarchShape m = untuned_shape(c, plan, dflt);
-    if (m.L != 0 || !plan || !plan->tev[0]) return launch_march_planned(ma, c, m, plan, dflt.L, s);
+    const MarchShape first = untuned_shape(c, plan, dflt);
+    if (!plan || !plan->tev[0]) return launch_march_planned(ma, c, first, plan, dflt.L, s);
+    MarchShape cand[kTrialMaxCand];
+    int n = 0;
+    cand[n++] = first;
+    if (first.L == 0) cand[n++] = dflt;
+    const MarchShape narrow{4, c.nz < 32 ? c.nz : 32, 2};
+    if (c.q == 19 && !(dflt.ty == narrow.ty && dflt.L == narrow.L)) cand[n++] = narrow;
+    auto run = [&](const MarchShape &m) { return launch_march_planned(ma, c, m, plan, dflt.L, s); };
     hipError_t e = hipSuccess;
     switch (plan->trial) {
-    case kTrialKeepUniform:
-        return launch_march_args(ma, c.q, dflt, s);
-    case kTrialUniform:
-        e = launch_march_args(ma, c.q, dflt, s);
-        if (e == hipSuccess && --plan->trial_left == 0) {
-            e = hipEventRecord(plan->tev[3], s);
-            plan->trial = kTrialWait;
-        }
-        return e;
-    case kTrialWait:
-        if (hipEventQuery(plan->tev[3]) != hipSuccess) {
-            // not complete yet: the query's status must not surface as the next launch's error
-            (void)hipGetLastError();
-        } else {
-            float tp = 0.0f, tu = 0.0f;
-            const bool ok = hipEventElapsedTime(&tp, plan->tev[0], plan->tev[1]) == hipSuccess &&
-                            hipEventElapsedTime(&tu, plan->tev[2], plan->tev[3]) == hipSuccess;
-            plan->trial = (ok && tu < tp * 1.02f) ? kTrialKeepUniform : kTrialKeepPlanned;
-            if (plan->trial == kTrialKeepUniform) return launch_march_args(ma, c.q, dflt, s);
-        }
-        break;
+    case kTrialKeep:
+        if (++plan->trial_since < kTrialPeriod) return run(plan->keep);
+        plan->trial = kTrialNone;  // race again: the interface has moved
+        if (plan->keep.L != 0) plan->ty = 0;  // planned chunks race on a fresh plan
+        return run(plan->keep);
     case kTrialNone:
-        // start on a corrected plan, with no re-plan due within the timed steps
-        if (plan->ty == m.ty && plan->since + kTrialSteps < plan->period) {
-            e = hipEventRecord(plan->tev[0], s);
-            if (e != hipSuccess) return e;
-            plan->trial = kTrialPlanned;
-            plan->trial_left = kTrialSteps;
-        }
+        // start once a planned first candidate has its corrected plan, with no re-plan due within
+        // its timed steps
+        if (first.L == 0 && !(plan->ty == first.ty && plan->since + kTrialSteps < plan->period)) return run(first);
+        plan->trial = kTrialTiming;
+        plan->trial_cand = 0;
+        plan->trial_left = kTrialSteps;
+        e = hipEventRecord(plan->tev[0], s);
+        if (e != hipSuccess) return e;
         break;
     default:
         break;
     }
-    e = launch_march_planned(ma, c, m, plan, dflt.L, s);
-    if (e == hipSuccess && plan->trial == kTrialPlanned && --plan->trial_left == 0) {
-        e = hipEventRecord(plan->tev[1], s);
-        if (e == hipSuccess) e = hipEventRecord(plan->tev[2], s);
-        plan->trial = kTrialUniform;
+    // kTrialTiming: one step of the candidate being timed
+    const int k = plan->trial_cand < n ? plan->trial_cand : n - 1;
+    e = run(cand[k]);
+    if (e != hipSuccess || --plan->trial_left > 0) return e;
+    e = hipEventRecord(plan->tev[2 * k + 1], s);
+    if (e != hipSuccess) return e;
+    if (k + 1 < n) {
+        plan->trial_cand = k + 1;
         plan->trial_left = kTrialSteps;
+        return hipEventRecord(plan->tev[2 * k + 2], s);
     }
-    return e;
+    // Every candidate timed: wait for the last event (the host would otherwise queue many steps of
+    // one candidate before the result is known) and keep the fastest.
+    e = hipEventSynchronize(plan->tev[2 * k + 1]);
+    if (e != hipSuccess) return e;
+    int best = 0;
+    float best_ms = 0.0f;
+    for (int j = 0; j < n; ++j) {
+        float ms = 0.0f;
+        e = hipEventElapsedTime(&ms, plan->tev[2 * j], plan->tev[2 * j + 1]);
+        if (e != hipSuccess) return e;
+        if (cand[j].L == 0) ms *= 1.02f;
+        if (j == 0 || ms < best_ms) {
+            best = j;
+            best_ms = ms;
+        }
+    }
+    plan->keep = cand[best];
+    plan->trial = kTrialKeep;
+    plan->trial_since = 0;
+    return hipSuccess;
 }
 }  // namespace
 

@@ -22,6 +22,7 @@ BcArgs make_bc_args(const cglbm_moments *B, const cglbm_moments *Aold, const cgl
 // Balanced z-chunk plan of a library-owned domain (plan.hip). A marching shape with L == 0 asks
 // for it: the step after every `period` steps (and the first) records the interface map and is
 // followed by the planner; the steps in between take their chunks from `items`.
+inline constexpr int kTrialMaxCand = 3;
 struct MarchPlanState {
     int4 *items = nullptr;    // cap work items
     uint8_t *wmap = nullptr;  // interface map of the last recording step (plan_wmap_bytes)
@@ -31,24 +32,26 @@ struct MarchPlanState {
     int ty = 0;               // tile rows the current plan was made for; 0 = none yet
     int period = 64;
     int since = 0;            // steps since the plan was made
-    // Planned or uniform chunks, for a domain left to the default shape: once the first plan and
-    // its measured correction are made, kTrialSteps planned steps and kTrialSteps steps of the
-    // uniform default are timed with events on the domain's stream; the result is read without
-    // blocking when the events have completed, and the faster form is kept until the state is
-    // replaced. Results are bitwise the same either way. No events (caller plans): no trial.
-    hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // Shape race of a domain left to the default shape: planned chunks (where they apply, once the
+    // first plan and its measured correction are made) and the uniform candidates each run
+    // kTrialSteps steps between two events on the domain's stream; the host then waits for the last
+    // event and keeps the fastest shape for kTrialPeriod steps, after which the race runs again
+    // (the interface moves), or until the state is replaced.
+    // Results are bitwise the same for every shape. No events (caller plans): no race.
+    hipEvent_t tev[2 * kTrialMaxCand] = {};
     int trial = 0;            // kTrial* below
-    int trial_left = 0;       // steps left in the current timed phase
+    int trial_cand = 0;       // candidate being timed
+    int trial_left = 0;       // steps left in its timed phase
+    int64_t trial_since = 0;  // steps since the last decision
+    MarchShape keep{0, 0, 0}; // shape kept by the last race (ty == 0: none yet)
 };
 enum : int {
-    kTrialNone = 0,     // not measured yet (planned steps meanwhile)
-    kTrialPlanned = 1,  // timing planned steps
-    kTrialUniform = 2,  // timing uniform steps
-    kTrialWait = 3,     // both timed, result pending (planned steps meanwhile)
-    kTrialKeepPlanned = 4,
-    kTrialKeepUniform = 5,
+    kTrialNone = 0,    // not raced yet (the first candidate's steps meanwhile)
+    kTrialTiming = 1,  // timing candidate trial_cand
+    kTrialKeep = 2,    // `keep` decided
 };
 inline constexpr int kTrialSteps = 3;
+inline constexpr int64_t kTrialPeriod = 2048;
 // Work items of a planned step for tiles of `ty` rows: one per CU, rounded up to a multiple of
 // the CU count so every column tile gets at least one.
 int plan_items(const cglbm_config &c, int ty, int ncu);
