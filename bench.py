@@ -362,6 +362,46 @@ def cpu_baseline(budget_s: float = 4.0):
             "sample": f"{k} steps of the same D3Q19 128^3 drop with the oracle", **info}
 
 
+SPLIT_CASES = (
+    # (label, q, storage, state, tile rows, chunk planes (0 = planned), exchange buffers)
+    ("flat_q19_ty16_L128", 19, 0, "flat", 16, 128, 2),
+    ("flat_q19_ty16_L16", 19, 0, "flat", 16, 16, 2),
+    ("flat_q19_ty8_L128", 19, 0, "flat", 8, 128, 2),
+    ("flat_q19_ty4_L128", 19, 0, "flat", 4, 128, 2),
+    ("drop_q19_ty16_L128", 19, 0, "drop", 16, 128, 2),
+    ("drop_q19_ty16_Lplan", 19, 0, "drop", 16, 0, 2),
+    ("flat_q27_ty16_L128", 27, 0, "flat", 16, 128, 1),
+    ("flat_q27_ty16_L16", 27, 0, "flat", 16, 16, 1),
+    ("drop_q27_ty16_L128", 27, 0, "drop", 16, 128, 1),
+    ("drop_q27_ty16_Lplan", 27, 0, "drop", 16, 0, 1),
+    ("flat_f16_q19_ty16_L128", 19, 1, "flat", 16, 128, 2),
+    ("drop_f16_q19_ty16_Lplan", 19, 1, "drop", 16, 0, 2),
+)
+
+
+def valu_split(cg, n: int = 128):
+    """Launches for an instruction-counter profile of the marching kernel (run under rocprofv3
+    --pmc SQ_INSTS_VALU ...): the same kernel on a state with no interface (rho = 1, one phase) and
+    on the 128^3 drop, with one chunk per column tile (no chunk edges) or 16-plane or planned
+    chunks, and 16-, 8- and 4-row tiles, so that per-update VALU work splits into own-wave bulk
+    work, halo (ring) work, the interface form and chunk edges. A few steps each, in the order of
+    SPLIT_CASES; the dispatch order identifies the cases."""
+    import numpy as np
+    for label, q, st, state, ty, L, nb in SPLIT_CASES:
+        cfg = cg.Config(nx=n, ny=n, nz=n, q=q, tau=0.8, sigma=0.01, beta=0.7)
+        with cg.Domain(cfg, storage=cg.STORE_F16 if st else cg.STORE_F32) as d:
+            if state == "flat":
+                S = np.zeros(cfg.shape, np.float32)
+                S[0] = 1.0
+                d.set_state(S)
+            else:
+                d.init_bubble(radius=24.0 * n / 128.0, width=4.0)
+            d.set_tile(ty, L, nb)
+            d.step(8 if L == 0 else 3)
+            d.sync()
+        print(f"bench: split case {label} done", file=sys.stderr, flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -381,12 +421,17 @@ def main() -> int:
     ap.add_argument("--tile27", default=None, help="fixed D3Q27 tile 'rows,chunk,buffers' instead of autotuning")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--valu-split", action="store_true",
+                    help="only the launches of the VALU split profile (profiles/march_bubble128.md), no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the update of one domain "
               f"runs on one GPU; multi-GPU sharding is a design note, not a built path)", file=sys.stderr)
         return 2
     cg = _load()
+    if args.valu_split:
+        valu_split(cg, args.n)
+        return 0
     kern = {"auto": cg.KERNEL_AUTO, "gather": cg.KERNEL_GATHER, "march": cg.KERNEL_MARCH}[args.kernel]
     print(f"bench: D3Q19 {args.n}^3 warmup {args.warmup} steps {args.steps}", file=sys.stderr, flush=True)
     t19 = [int(v) for v in args.tile19.split(",")] if args.tile19 else None
