@@ -481,14 +481,15 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     __syncthreads();
 
     const int kend = Lc + 1;
-    for (int k = 0; k <= kend; ++k) {
-        const int p = z0 - 1 + k;  // source plane of this step; target plane p - 1 completes
+    // One step: source plane p = z0 - 1 + k, target plane p - 1 completes. OWN selects the own
+    // waves' code at compile time; HA, HB, HC hold the populations received so far by targets
+    // p - 1, p and p + 1 (own waves only).
+    auto step = [&](int k, auto OWN, Held<Q> &HA, Held<Q> &HB, Held<Q> &HC) {
+        constexpr bool own = decltype(OWN)::value;
+        const int p = z0 - 1 + k;
         const int pz = wrapz(p);
         const bool edge = (k == 0) || (k == kend);
         const int xb = NBUF == 2 ? (k & 1) : 0;
-        Held<Q> &HA = H0;  // target p - 1
-        Held<Q> &HB = H1;  // target p
-        Held<Q> &HC = H2;  // target p + 1
 #if CGLBM_MARCH_PREFETCH
         const Mom cur = nxt;
         if (k < kend && srow >= 0) nxt = Abuf.mom(cell(gx, srow, wrapz(p + 1)));
@@ -549,7 +550,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
                     }
                 }
             };
-            if (role == kRoleOwn) {
+            if constexpr (own) {
                 if (edge) run(std::integral_constant<uint32_t, MZ>{}, std::false_type{}, own_sink);
                 else run(std::integral_constant<uint32_t, ALL>{}, std::true_type{}, own_sink);
             } else if (RW == 2) {
@@ -579,7 +580,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
             }
         }
         if constexpr (REC) {
-            if (role == kRoleOwn && !edge) {  // wave-uniform condition
+            if (own && !edge) {  // wave-uniform condition
                 const bool any = __builtin_amdgcn_ballot_w64(ion) != 0;
                 if ((t & 63) == 0) {
                     const int ntx = (nx + TX - 1) / TX;
@@ -593,13 +594,13 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
         // ---- ring lanes: psi two planes ahead (plane p + 3, first read at step k + 2) ------
         // (loading it before the sources instead, to hide the load, measured no faster on
         // single domains and 2.4% slower on batches: the registers it holds across the sources)
-        if (ring && k + 3 <= kend + 1) {
+        if (!own && k + 3 <= kend + 1) {
             psi_load(p + 3);
             psi_store(psi_slot(p + 3));
         }
 
         // ---- targets ----------------------------------------------------------------------
-        if (role == kRoleOwn) {
+        if constexpr (own) {
             unroll<0, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
                 if constexpr (cross_col(i)) {
@@ -618,10 +619,23 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
                 for (int f = 0; f < 11; ++f) Bbuf.st(f, id, out[f]);
                 if constexpr (YBC != kYPeriodic) boundary_planes(a, Abuf, Bbuf, gx, zt, rsrc, out);
             }
-            H0 = H1;  // target p becomes p - 1, p + 1 becomes p
-            H1 = H2;
         }
         if constexpr (NBUF == 1) __syncthreads();
+    };
+    // Own waves: after a step, target p becomes p - 1 and p + 1 becomes p, so the three sets
+    // rotate with period 3; the loop is unrolled by three with the sets renamed instead of
+    // copied. Ring waves hold no targets. Both loops run kend + 1 steps and so the same
+    // barriers.
+    if (role == kRoleOwn) {
+        for (int k = 0; k <= kend; k += 3) {
+            step(k, std::true_type{}, H0, H1, H2);
+            if (k + 1 > kend) break;
+            step(k + 1, std::true_type{}, H1, H2, H0);
+            if (k + 2 > kend) break;
+            step(k + 2, std::true_type{}, H2, H0, H1);
+        }
+    } else {
+        for (int k = 0; k <= kend; ++k) step(k, std::false_type{}, H0, H1, H2);
     }
     if constexpr (NBUF == 2) __syncthreads();
 }
