@@ -423,6 +423,75 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
     });
 }
 
+// One member of each pair in PMASK, for sources of which only one direction of a pair reaches a
+// target that is accumulated (halo sources beside the tile, the first and last source plane of a
+// chunk): direction I = 2k + 1 when bit k of OMASK is clear, its opposite O = I + 1 when it is
+// set, and the other one of the two for a lane whose `flip` is 0x80000000 (flip 0 keeps the
+// choice). Member O is member I's expressions with c.u, the third-order regularized term and the
+// recolouring term negated: emit_regular forms O from the same intermediates with these exact sign
+// flips (-t1 for t1, -A3, An2 - An3, go - D), x + (-y) is x - y in binary32, and a product or
+// quotient with one operand negated is the negated result, so every population is bitwise the one
+// emit_regular hands to the sink. sink(integral_constant<I>, integral_constant<bool, OMASK bit>,
+// fr, fb); the member the lane took is O exactly when that bit differs from `flip`. No body-force
+// form (the forced kernels emit both members).
+template <int Q, uint32_t PMASK, uint32_t OMASK, bool IFACE, typename Sink>
+__device__ __forceinline__ void emit_one(const Mom &s, const Coeffs &c, const Iface &fi, uint32_t flip, Sink &&sink) {
+    const float rho = s.rr + s.rb;
+    const float inv = 1.0f / (s.rr + s.rb);
+    const float aR = s.rr * inv, aB = s.rb * inv;
+    SrcTerms<Q> t;
+    t.build_feq(rho, s.ux, s.uy, s.uz);
+    t.build_neq(s.pxx, s.pxy, s.pyy, s.pyz, s.pzz, s.pxz, s.ux, s.uy, s.uz);
+    unroll<0, (Q - 1) / 2>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        if constexpr ((PMASK >> k) & 1u) {
+            constexpr int I = 2 * k + 1;
+            constexpr bool OM = (OMASK >> k) & 1u;
+            constexpr float w = weight<Q>(I);
+            const uint32_t sg = flip ^ (OM ? 0x80000000u : 0u);
+            const auto neg = [sg](float v) { return __uint_as_float(__float_as_uint(v) ^ sg); };
+            const float wr = t.template wr<I>();
+            const float cu = neg(cdot<I>(s.ux, s.uy, s.uz));
+            const float t1 = cu * kInvCs2;
+            const float cu2 = cu * cu;
+            const float t3 = cu2 * kInv2Cs4;
+            const float A2 = (t1 - t.T2) + t3;
+            const float inner = cu2 * kInv6Cs6 - t.U4;
+            const float A3 = cu * inner;
+            const float feq = wr * ((1.0f + A2) + A3);
+            const float An2 = a2neq<Q, I>(t);
+            const float An3 = neg(a3neq<Q, I>(t));
+            float g = feq + c.oms * (w * (An2 + An3));
+            const auto IC = std::integral_constant<int, I>{};
+            const auto OC = std::integral_constant<bool, OM>{};
+            if constexpr (IFACE && Q != 19) {
+                const float Fc = cdot<I>(fi.Fx, fi.Fy, fi.Fz);
+                const float c2q = (Fc * Fc) / fi.af2;
+                const float c2 = fi.small ? 0.0f : c2q;
+                g += fi.omega ? fi.ha * (w * c2 - bcoef<Q>(I)) : -0.0f;
+                const float cq = Fc * invcnorm(I) / fi.af;
+                const float cphi = fi.small ? 0.0f : cq;
+                const float D = neg(fi.coeff * cphi * wr);
+                sink(IC, OC, aR * g + D, aB * g - D);
+            } else if constexpr (IFACE) {
+                const float Fc = cdot<I>(fi.Fx, fi.Fy, fi.Fz);
+                const float c2q = div_markstein(Fc * Fc, fi.af2, fi.rinv_af2);
+                const float c2 = fi.small ? 0.0f : c2q;
+                g += fi.omega ? fi.ha * (w * c2 - bcoef<Q>(I)) : -0.0f;
+                const float cq = div_shared(Fc * invcnorm(I), fi.af, fi.rinv_af);
+                const float cphi = fi.small ? 0.0f : cq;
+                const float D = neg(fi.coeff * cphi * wr);
+                // as emit_regular: a non-interface lane of an interface wave adds -0 / subtracts +0
+                const float Dp = fi.on ? D : -0.0f;
+                const float Dm = fi.on ? D : 0.0f;
+                sink(IC, OC, aR * g + Dp, aB * g - Dm);
+            } else {
+                sink(IC, OC, aR * g, aB * g);
+            }
+        }
+    });
+}
+
 // Ghost-plane source (lbm.cuh:79-151): equilibrium from the ghost's rho and u, regularized part
 // from the adjacent interior cell F; no Omega2, recolouring or body force.
 template <int Q, uint32_t PMASK, bool REST, typename Sink>

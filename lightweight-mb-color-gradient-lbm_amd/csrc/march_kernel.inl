@@ -76,6 +76,18 @@ template <int TY>
 __host__ __device__ constexpr int block_threads() {
     return TX * TY + 64 * ring_waves<TY>();
 }
+// Pairs whose member I = 2k + 1 has a negative component along `axis`: for them the member with
+// the positive component is O = I + 1 (the OMASK of emit_one).
+template <int Q>
+__host__ __device__ constexpr uint32_t pair_mask_neg(int axis) {
+    uint32_t m = 0;
+    for (int k = 0; k < (Q - 1) / 2; ++k) {
+        const int i = 2 * k + 1;
+        const int c = axis == 0 ? kCx[i] : (axis == 1 ? kCy[i] : kCz[i]);
+        if (c < 0) m |= 1u << k;
+    }
+    return m;
+}
 // Half h (0 or 1) of the pairs in mask m: the first ceil(n/2) set bits, or the rest.
 __host__ __device__ constexpr uint32_t mask_half(uint32_t m, int h) {
     int n = 0;
@@ -311,6 +323,17 @@ __device__ __forceinline__ void boundary_planes(const MarchArgs &a, const FieldB
 
 enum : int { kRoleOwn = 0, kRoleRows = 1, kRoleCols = 2, kRoleNone = 3 };
 
+// v mod n (result in [0, n)) for the indices of a tile and its halo, which lie within one
+// extent of [0, n): one conditional add or subtract. Extents smaller than the tile can put v
+// further out; those lanes take the general remainder (a branch no lane of a larger domain takes,
+// so the integer division it needs is not executed there).
+__device__ __forceinline__ int wrap_near(int v, int n) {
+    if (v < 0) v += n;
+    else if (v >= n) v -= n;
+    if (v < 0 || v >= n) v = ((v % n) + n) % n;
+    return v;
+}
+
 // One work item: tile (tix, tiy), target planes z0 .. z1-1 (z0 < z1), of the domain described by
 // `a`, executed by the whole work-group. Ends with the work-group synchronised so the caller may
 // run another item with the same LDS. REC: also write the interface map a.wmap (a separate
@@ -370,13 +393,13 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
         }
     }
     const bool ring = t >= NOWN;
-    const int gx = ((x0 + sx) % nx + nx) % nx;
+    const int gx = wrap_near(x0 + sx, nx);
     const int rsrc = r0 + sy;  // interior row of the source
     int srow = -1;             // stored row of the source, -1 = not needed
     bool sghost = false;
     if (role != kRoleNone) {
         if constexpr (YBC == kYPeriodic) {
-            srow = ((rsrc % ny) + ny) % ny;
+            srow = wrap_near(rsrc, ny);
         } else if (rsrc >= -1 && rsrc <= ny) {
             srow = rsrc + 1;
             sghost = (rsrc == -1 || rsrc == ny);
@@ -388,7 +411,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     auto cell = [&](int x, int row, int z) -> uint32_t {
         return static_cast<uint32_t>(x) + a.py * static_cast<uint32_t>(row) + a.pz * static_cast<uint32_t>(z);
     };
-    auto wrapz = [&](int z) { return ((z % nz) + nz) % nz; };
+    auto wrapz = [&](int z) { return wrap_near(z, nz); };
 
     // ---- psi cells of a ring lane ----------------------------------------------------------
     int psi_x[NPASS], psi_row[NPASS];
@@ -397,10 +420,10 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     for (int k = 0; k < NPASS; ++k) {
         const int ci = (t - NOWN) + k * NRING;
         const int px = ci % PX - 2, py = ci / PX - 2;
-        psi_x[k] = ((x0 + px) % nx + nx) % nx;
+        psi_x[k] = wrap_near(x0 + px, nx);
         const int rr_ = r0 + py;
         if (!ring || ci >= NPSI) psi_row[k] = -2;
-        else if constexpr (YBC == kYPeriodic) psi_row[k] = ((rr_ % ny) + ny) % ny;
+        else if constexpr (YBC == kYPeriodic) psi_row[k] = wrap_near(rr_, ny);
         else psi_row[k] = (rr_ >= -1 && rr_ <= ny) ? rr_ + 1 : -1;
         psi_r[k] = 0.0f;
         psi_b[k] = 1.0f;
@@ -550,32 +573,80 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
                     }
                 }
             };
+            // one member per pair (emit_one): the halo sources push only the member that enters the
+            // tile (c_y toward the tile for the row ring, c_x for the column ring: a lane on the far
+            // side takes the other member), the first and last source plane of a chunk only the
+            // member that reaches the chunk's first or last target plane (c_z = +1, -1)
+            const uint32_t ring_flip = (role == kRoleRows ? sy == TY : sx == TX) ? 0x80000000u : 0u;
+            auto ring_push1 = [&](auto I, auto OMC, float fr, float fb) {
+                constexpr int i = decltype(I)::value, o = i + 1;
+                static_assert(cross_col(i) && cross_col(o), "ring pairs leave the column");
+                const bool takeO = decltype(OMC)::value != (ring_flip != 0u);
+                const int txx = sx + (takeO ? kCx[o] : kCx[i]), tyy = sy + (takeO ? kCy[o] : kCy[i]);
+                if (txx >= -MG && txx < TX + MG && tyy >= -MG && tyy < TY + MG)
+                    sh.xfer[xb][XL::at(takeO ? slot_of(o) : slot_of(i), txx, tyy)] = make_float2(fr, fb);
+            };
+            auto own_push1 = [&](auto I, auto OMC, float fr, float fb) {
+                constexpr int d = decltype(I)::value + (decltype(OMC)::value ? 1 : 0);
+                own_sink(std::integral_constant<int, d>{}, fr, fb);
+            };
+            auto run1 = [&](auto MASKC, auto OMASKC, uint32_t flip, auto &sink3, auto &sink1) {
+                constexpr uint32_t M = decltype(MASKC)::value;
+                constexpr uint32_t OM = decltype(OMASKC)::value;
+                if (YBC != kYPeriodic && sghost) {
+                    emit_ghost<Q, M, false>(cur, curF, c, sink3);
+                } else {
+                    Iface f = iface_begin(cur);
+                    const bool iface = Q == 19 ? (__builtin_amdgcn_ballot_w64(f.on) != 0) : f.on;
+                    if (iface) {
+                        colour_gradient<Q>(psi_reader(p), f.Fx, f.Fy, f.Fz);
+                        iface_finish<Q>(f, cur, c, a.conservative);
+                        emit_one<Q, M, OM, true>(cur, c, f, flip, sink1);
+                    } else {
+                        emit_one<Q, M, OM, false>(cur, c, f, flip, sink1);
+                    }
+                }
+            };
+            constexpr uint32_t NX_ = pair_mask_neg<Q>(0), NY_ = pair_mask_neg<Q>(1), NZ_ = pair_mask_neg<Q>(2);
+            auto ring_run = [&](auto MASKC, auto OMASKC) {
+                if constexpr (FORCED) run(MASKC, std::false_type{}, lds_push);
+                else run1(MASKC, OMASKC, ring_flip, lds_push, ring_push1);
+            };
             if constexpr (own) {
-                if (edge) run(std::integral_constant<uint32_t, MZ>{}, std::false_type{}, own_sink);
-                else run(std::integral_constant<uint32_t, ALL>{}, std::true_type{}, own_sink);
+                if (!edge) {
+                    run(std::integral_constant<uint32_t, ALL>{}, std::true_type{}, own_sink);
+                } else if constexpr (FORCED) {
+                    run(std::integral_constant<uint32_t, MZ>{}, std::false_type{}, own_sink);
+                } else if (k == 0) {  // first source plane: the c_z = +1 members
+                    run1(std::integral_constant<uint32_t, MZ>{}, std::integral_constant<uint32_t, MZ & NZ_>{}, 0u,
+                         own_sink, own_push1);
+                } else {  // last source plane: the c_z = -1 members
+                    run1(std::integral_constant<uint32_t, MZ>{}, std::integral_constant<uint32_t, MZ & ~NZ_>{}, 0u,
+                         own_sink, own_push1);
+                }
             } else if (RW == 2) {
                 if (role == kRoleRows) {
-                    if (edge) run(std::integral_constant<uint32_t, MY & MZ>{}, std::false_type{}, lds_push);
-                    else run(std::integral_constant<uint32_t, MY>{}, std::false_type{}, lds_push);
+                    if (edge) ring_run(std::integral_constant<uint32_t, MY & MZ>{}, std::integral_constant<uint32_t, NY_>{});
+                    else ring_run(std::integral_constant<uint32_t, MY>{}, std::integral_constant<uint32_t, NY_>{});
                 } else {
-                    if (edge) run(std::integral_constant<uint32_t, MX & MZ>{}, std::false_type{}, lds_push);
-                    else run(std::integral_constant<uint32_t, MX>{}, std::false_type{}, lds_push);
+                    if (edge) ring_run(std::integral_constant<uint32_t, MX & MZ>{}, std::integral_constant<uint32_t, NX_>{});
+                    else ring_run(std::integral_constant<uint32_t, MX>{}, std::integral_constant<uint32_t, NX_>{});
                 }
             } else if (role == kRoleRows) {
                 if (half == 0) {
-                    if (edge) run(std::integral_constant<uint32_t, mask_half(MY & MZ, 0)>{}, std::false_type{}, lds_push);
-                    else run(std::integral_constant<uint32_t, mask_half(MY, 0)>{}, std::false_type{}, lds_push);
+                    if (edge) ring_run(std::integral_constant<uint32_t, mask_half(MY & MZ, 0)>{}, std::integral_constant<uint32_t, NY_>{});
+                    else ring_run(std::integral_constant<uint32_t, mask_half(MY, 0)>{}, std::integral_constant<uint32_t, NY_>{});
                 } else {
-                    if (edge) run(std::integral_constant<uint32_t, mask_half(MY & MZ, 1)>{}, std::false_type{}, lds_push);
-                    else run(std::integral_constant<uint32_t, mask_half(MY, 1)>{}, std::false_type{}, lds_push);
+                    if (edge) ring_run(std::integral_constant<uint32_t, mask_half(MY & MZ, 1)>{}, std::integral_constant<uint32_t, NY_>{});
+                    else ring_run(std::integral_constant<uint32_t, mask_half(MY, 1)>{}, std::integral_constant<uint32_t, NY_>{});
                 }
             } else {
                 if (half == 0) {
-                    if (edge) run(std::integral_constant<uint32_t, mask_half(MX & MZ, 0)>{}, std::false_type{}, lds_push);
-                    else run(std::integral_constant<uint32_t, mask_half(MX, 0)>{}, std::false_type{}, lds_push);
+                    if (edge) ring_run(std::integral_constant<uint32_t, mask_half(MX & MZ, 0)>{}, std::integral_constant<uint32_t, NX_>{});
+                    else ring_run(std::integral_constant<uint32_t, mask_half(MX, 0)>{}, std::integral_constant<uint32_t, NX_>{});
                 } else {
-                    if (edge) run(std::integral_constant<uint32_t, mask_half(MX & MZ, 1)>{}, std::false_type{}, lds_push);
-                    else run(std::integral_constant<uint32_t, mask_half(MX, 1)>{}, std::false_type{}, lds_push);
+                    if (edge) ring_run(std::integral_constant<uint32_t, mask_half(MX & MZ, 1)>{}, std::integral_constant<uint32_t, NX_>{});
+                    else ring_run(std::integral_constant<uint32_t, mask_half(MX, 1)>{}, std::integral_constant<uint32_t, NX_>{});
                 }
             }
         }
