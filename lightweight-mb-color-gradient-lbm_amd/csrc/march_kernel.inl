@@ -413,30 +413,51 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     };
     auto wrapz = [&](int z) { return wrap_near(z, nz); };
 
+    // Cell (x, row) the moment loads of this lane address. Every lane loads, so the loads carry no
+    // lane mask and overwrite their registers whole instead of merging into the previous plane's
+    // values. A lane with no source reads a cell the work-group loads anyway (its values are not
+    // used): an idle column-ring lane the first column-ring cell, a source row beyond the ghost
+    // planes of a bounded y the nearest ghost plane.
+    uint32_t lxy;
+    {
+        int ax = gx, arow = srow;
+        if (role == kRoleNone) {
+            ax = wrap_near(x0 - 1, nx);
+            arow = YBC == kYPeriodic ? r0 : r0 + 1;
+        } else if (srow < 0) {
+            arow = rsrc < -1 ? 0 : ny + 1;
+        }
+        lxy = static_cast<uint32_t>(ax) + a.py * static_cast<uint32_t>(arow);
+    }
+    auto lcell = [&](int z) -> uint32_t { return lxy + a.pz * static_cast<uint32_t>(z); };
+
     // ---- psi cells of a ring lane ----------------------------------------------------------
-    int psi_x[NPASS], psi_row[NPASS];
+    // psi_row: stored row of the psi cell, -1 outside a bounded y (psi 0), -2 no psi cell (no
+    // store). Every ring lane loads, unmasked as the moment loads: a lane past the psi ring reads
+    // the ring's last cell, a psi cell outside a bounded y the nearest ghost plane.
+    int psi_row[NPASS];
+    uint32_t psi_xy[NPASS];
     float psi_r[NPASS], psi_b[NPASS];
 #pragma unroll
     for (int k = 0; k < NPASS; ++k) {
         const int ci = (t - NOWN) + k * NRING;
-        const int px = ci % PX - 2, py = ci / PX - 2;
-        psi_x[k] = wrap_near(x0 + px, nx);
+        const int ca = ci < NPSI ? ci : NPSI - 1;
+        const int px = ca % PX - 2, py = ca / PX - 2;
         const int rr_ = r0 + py;
+        int arow;
+        if constexpr (YBC == kYPeriodic) arow = wrap_near(rr_, ny);
+        else arow = rr_ < -1 ? 0 : (rr_ > ny ? ny + 1 : rr_ + 1);
+        psi_xy[k] = static_cast<uint32_t>(wrap_near(x0 + px, nx)) + a.py * static_cast<uint32_t>(arow);
         if (!ring || ci >= NPSI) psi_row[k] = -2;
-        else if constexpr (YBC == kYPeriodic) psi_row[k] = wrap_near(rr_, ny);
-        else psi_row[k] = (rr_ >= -1 && rr_ <= ny) ? rr_ + 1 : -1;
-        psi_r[k] = 0.0f;
-        psi_b[k] = 1.0f;
+        else if constexpr (YBC == kYPeriodic) psi_row[k] = arow;
+        else psi_row[k] = (rr_ >= -1 && rr_ <= ny) ? arow : -1;
     }
     auto psi_load_into = [&](int z, float (&r)[NPASS], float (&b)[NPASS]) {
-        const int zz = wrapz(z);
+        const uint32_t zo = a.pz * static_cast<uint32_t>(wrapz(z));
 #pragma unroll
         for (int k = 0; k < NPASS; ++k) {
-            if (psi_row[k] >= 0) {
-                const uint32_t id = cell(psi_x[k], psi_row[k], zz);
-                r[k] = Abuf.ld(0, id);
-                b[k] = Abuf.ld(1, id);
-            }
+            r[k] = Abuf.ld(0, psi_xy[k] + zo);
+            b[k] = Abuf.ld(1, psi_xy[k] + zo);
         }
     };
     auto psi_store_from = [&](int slot, const float (&r)[NPASS], const float (&b)[NPASS]) {
@@ -483,22 +504,15 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     if (ring) {
         float pr[4][NPASS], pb[4][NPASS];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-#pragma unroll
-            for (int k = 0; k < NPASS; ++k) {
-                pr[q][k] = 0.0f;
-                pb[q][k] = 1.0f;
-            }
-            psi_load_into(z0 - 2 + q, pr[q], pb[q]);
-        }
+        for (int q = 0; q < 4; ++q) psi_load_into(z0 - 2 + q, pr[q], pb[q]);
 #if CGLBM_MARCH_PREFETCH
-        if (srow >= 0) nxt = Abuf.mom(cell(gx, srow, wrapz(z0 - 1)));
+        nxt = Abuf.mom(lcell(wrapz(z0 - 1)));
 #endif
 #pragma unroll
         for (int q = 0; q < 4; ++q) psi_store_from(q, pr[q], pb[q]);
     } else {
 #if CGLBM_MARCH_PREFETCH
-        if (srow >= 0) nxt = Abuf.mom(cell(gx, srow, wrapz(z0 - 1)));
+        nxt = Abuf.mom(lcell(wrapz(z0 - 1)));
 #endif
     }
     __syncthreads();
@@ -514,15 +528,20 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
         const bool edge = (k == 0) || (k == kend);
         const int xb = NBUF == 2 ? (k & 1) : 0;
 #if CGLBM_MARCH_PREFETCH
+        // The next plane's moments are prefetched; the last step loads its own plane again (cached
+        // and unused): an unconditional load keeps the prefetch out of a merge with the previous
+        // values, which would wait for the loads.
         const Mom cur = nxt;
-        if (k < kend && srow >= 0) nxt = Abuf.mom(cell(gx, srow, wrapz(p + 1)));
+        nxt = Abuf.mom(lcell(wrapz(k < kend ? p + 1 : p)));
 #else
-        Mom cur{};
-        if (srow >= 0) cur = Abuf.mom(cell(gx, srow, pz));
+        const Mom cur = Abuf.mom(lcell(pz));
 #endif
-        Mom curF{};
+        // the adjacent interior cell of a ghost source: loaded by every lane of a wave that holds a
+        // ghost source (unmasked, as above); read only by ghost sources
+        Mom curF;
         if constexpr (YBC != kYPeriodic) {
-            if (sghost) curF = Abuf.mom(cell(gx, frow, pz));
+            if (__builtin_amdgcn_ballot_w64(sghost) != 0)
+                curF = Abuf.mom(sghost ? cell(gx, frow, pz) : lcell(pz));
         }
 
         // ---- sources ----------------------------------------------------------------------
