@@ -216,11 +216,12 @@ struct Iface {
 
 // a / b correctly rounded, given y = RN(1 / b) shared by many divisions by b. Markstein's
 // correction: with q = RN(a y) and the exact remainder r = a - b q (one fma), RN(q + r y) is the
-// correctly rounded quotient provided r does not underflow. Zero numerators return a itself
-// (a / b for finite nonzero b, sign included); numerators below 2^-100, where the remainder could
-// underflow, take the IEEE division. A NaN y (b NaN) keeps the NaN. Three operations instead of
-// the ten of the IEEE division sequence; bitwise equal to a / b (checked against the IEEE
-// quotient for every significand of b and the operand ranges of the colour-gradient terms).
+// correctly rounded quotient provided r does not underflow. The quotient of positive b takes the
+// sign of a, which the corrected value has for every nonzero a; zero numerators, for which the
+// correction returns +0, get a's sign of zero back that way (a / b for finite nonzero b). Numerators
+// below 2^-100, where the remainder could underflow, take the IEEE division. Four operations
+// instead of the eleven of the IEEE division sequence; bitwise equal to a / b (checked against the
+// IEEE quotient for every significand of b and the operand ranges of the colour-gradient terms).
 // The tiny-numerator case is a wave-uniform branch that stays a branch: left to itself the
 // compiler evaluates the IEEE division on every call and selects, which costs more than the
 // division it replaces; the empty volatile asm cannot be speculated, so the division runs only in
@@ -228,8 +229,7 @@ struct Iface {
 __device__ __forceinline__ float div_shared(float a, float b, float y) {
     float q = a * y;
     const float r = fmaf(-q, b, a);
-    q = fmaf(r, y, q);
-    q = (a == 0.0f && y == y) ? a : q;
+    q = __builtin_copysignf(fmaf(r, y, q), a);
     const bool tiny = fabsf(a) < 0x1p-100f && a != 0.0f;
     if (__builtin_amdgcn_ballot_w64(tiny) != 0) {
         asm volatile("");

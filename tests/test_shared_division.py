@@ -20,8 +20,7 @@ static uint64_t nxt(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return s; }
 static float shared_div(float a, float b, float y) {
     float q = a * y;
     float r = fmaf(-q, b, a);
-    q = fmaf(r, y, q);
-    if (a == 0.0f && y == y) q = a;
+    q = copysignf(fmaf(r, y, q), a);
     if (fabsf(a) < 0x1p-100f && a != 0.0f) q = a / b;
     return q;
 }
