@@ -249,3 +249,57 @@ def test_inlet_perturbation_bounds(oracle):
     assert np.all(np.abs(B[2, :, 0][m]) <= U * eps * 1.0001)
     C = oracle.step(p, A, step_index=8)
     assert not np.array_equal(C[3, :, 0], uy)  # keyed by step
+
+
+def _interface_height(A, width):
+    """Interface height y_i(x, z) of an RTI state from its phase field: in each column, at the
+    stored row where |rb - rr| is smallest, phi = tanh((y - y_i)/w) with phi = rb - rr gives
+    y_i = y - w atanh(phi)."""
+    phi = (A[1].astype(np.float64) - A[0].astype(np.float64))  # (nz, nyt, nx)
+    row = np.argmin(np.abs(phi), axis=1)  # (nz, nx)
+    ph = np.take_along_axis(phi, row[:, None, :], axis=1)[:, 0, :]
+    return row - width * np.arctanh(ph)
+
+
+def test_rti_random_modes_spectrum(oracle):
+    """The RTI initialiser's seeded random modes (an addition of this build; the reference's
+    initialiser has the single cosine mode only): the interface height they add is a sum of
+    n_random_modes cosines of the given amplitude, with wave numbers 1..8 along x and along z and
+    uniform phases. Read back from the phase field, independently of how the initialiser builds it:
+    the added height has zero mean, its Fourier spectrum lives on the (kx, kz) in 1..8 (and their
+    conjugates) only, each isolated mode has the stated amplitude, the phases spread over the
+    circle and the wave numbers over 1..8; one seed reproduces its field, another changes it."""
+    n, width, ra = 32, 4.0, 0.2
+    p = oracle.Params(nx=n, ny=48, nz=n, q=19, ybc=oracle.Y_WALLS, tau=0.6, sigma=0.002, beta=0.7)
+    base = _interface_height(oracle.init_rti(p, 2.0, width, 0, 0.0, 0), width)
+    amps, phases, waves = [], [], []
+    for seed in range(40):
+        A = oracle.init_rti(p, 2.0, width, 8, ra, seed)
+        d = _interface_height(A, width) - base  # (nz, nx)
+        assert abs(d.mean()) < 1e-4
+        F = np.fft.fft2(d) / d.size  # a cos(2 pi (m x + k z)/n + ph) -> a/2 at (k, m) and (-k, -m)
+        amp = 2.0 * np.abs(F)
+        big = np.argwhere(amp > 0.02)
+        for kz, kx in big:
+            kz_s = kz if kz <= n // 2 else kz - n
+            kx_s = kx if kx <= n // 2 else kx - n
+            # one quadrant pair only: both wave numbers positive in 1..8, or both negative
+            assert (1 <= kx_s <= 8 and 1 <= kz_s <= 8) or (-8 <= kx_s <= -1 and -8 <= kz_s <= -1), (seed, kx_s, kz_s)
+        pos = [(kz, kx) for kz, kx in big if 1 <= kx <= 8 and 1 <= kz <= 8]
+        assert 1 <= len(pos) <= 8
+        assert sum(amp[kz, kx] for kz, kx in pos) <= 8 * ra + 1e-3
+        if len(pos) == 8:  # no two draws share a wave vector: every mode stands alone
+            for kz, kx in pos:
+                assert abs(amp[kz, kx] - ra) < 2e-3, (seed, kx, kz, amp[kz, kx])
+                amps.append(amp[kz, kx])
+                phases.append(np.angle(F[kz, kx]))
+                waves += [kx, kz]
+        assert np.array_equal(A, oracle.init_rti(p, 2.0, width, 8, ra, seed))
+    assert len(amps) >= 80  # enough isolated modes among the 40 seeds to judge their spread
+    # phases uniform on the circle: every quarter holds a fair share
+    q = np.histogram(np.mod(phases, 2 * np.pi), bins=4, range=(0, 2 * np.pi))[0]
+    assert q.min() > 0.12 * len(phases), q
+    # wave numbers cover 1..8
+    w = np.bincount(np.asarray(waves), minlength=9)[1:]
+    assert w.min() > 0.04 * len(waves), w
+    assert not np.array_equal(oracle.init_rti(p, 2.0, width, 8, ra, 1), oracle.init_rti(p, 2.0, width, 8, ra, 2))
