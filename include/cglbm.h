@@ -48,7 +48,7 @@ enum {
     CGLBM_EINVAL = -1,   /* invalid argument or configuration */
     CGLBM_ENOMEM = -2,   /* device or host allocation failed / over the memory budget */
     CGLBM_EHIP = -3,     /* HIP runtime error */
-    CGLBM_EIO = -4,      /* file output failed */
+    CGLBM_EIO = -4,      /* file input or output failed */
     CGLBM_ESTATE = -5,   /* call not valid in the object's current state */
 };
 
@@ -226,6 +226,19 @@ int cglbm_sync(cglbm_domain *dom);
  * f; the selected fields are packed consecutively in `host`. Synchronous. */
 int cglbm_copy_out(cglbm_domain *dom, uint32_t field_mask, float *host);
 int cglbm_copy_in(cglbm_domain *dom, uint32_t field_mask, const float *host);
+
+/* Restartable snapshots (no counterpart in the reference, whose only output is the 5-field VTI).
+ * cglbm_snapshot_write stores the domain's configuration, storage format, step count and its full
+ * current state (11 fields, ghost planes included; FP16 states decoded exactly) in one file;
+ * cglbm_snapshot_read restores them into a domain created with the same configuration and
+ * storage, after which stepping gives bitwise the results of the uninterrupted run (the inlet
+ * perturbation's step index included). cglbm_snapshot_info reads a snapshot's configuration,
+ * storage and step count (any output pointer may be NULL), to create the domain to restore into.
+ * Files carry a checksum and this library's ABI version; a mismatch is CGLBM_EINVAL, a file that
+ * cannot be opened or written CGLBM_EIO. Synchronous. */
+int cglbm_snapshot_write(cglbm_domain *dom, const char *path);
+int cglbm_snapshot_read(cglbm_domain *dom, const char *path);
+int cglbm_snapshot_info(const char *path, cglbm_config *cfg, int32_t *storage, int64_t *steps);
 
 /* FP64 diagnostics of the current state over interior cells, fixed summation order:
  * out[0] = sum rhor, out[1] = sum rhob, out[2..4] = sum rho*u, out[5] = count of bad cells: a

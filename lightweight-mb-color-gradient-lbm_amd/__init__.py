@@ -41,6 +41,7 @@ EXPORTS = (
     "cglbm_launch_rcs_planned_noalloc",
     "cglbm_init_bubble", "cglbm_init_jet", "cglbm_init_rti", "cglbm_step", "cglbm_time_steps", "cglbm_sync",
     "cglbm_copy_out", "cglbm_copy_in", "cglbm_reduce", "cglbm_write_vti", "cglbm_write_vti_host",
+    "cglbm_snapshot_write", "cglbm_snapshot_read", "cglbm_snapshot_info",
     "cglbm_batch_create", "cglbm_batch_destroy", "cglbm_batch_add", "cglbm_batch_remove", "cglbm_batch_size",
     "cglbm_batch_set_tile", "cglbm_batch_set_graph", "cglbm_batch_step", "cglbm_batch_time_steps",
     "cglbm_batch_time_sequential", "cglbm_batch_autotune", "cglbm_batch_sync",
@@ -210,6 +211,10 @@ def lib():
         "cglbm_copy_in": ([D, ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
         "cglbm_reduce": ([D, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "cglbm_write_vti": ([D, ctypes.c_char_p], ctypes.c_int),
+        "cglbm_snapshot_write": ([D, ctypes.c_char_p], ctypes.c_int),
+        "cglbm_snapshot_read": ([D, ctypes.c_char_p], ctypes.c_int),
+        "cglbm_snapshot_info": ([ctypes.c_char_p, C, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)],
+                                ctypes.c_int),
         "cglbm_write_vti_host": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5,
                                  ctypes.c_int),
         "cglbm_batch_create": ([ctypes.POINTER(D)], ctypes.c_int),
@@ -366,6 +371,17 @@ def domain_bytes(cfg: Config, storage: int = STORE_F32) -> int:
     return int(lib().cglbm_domain_bytes(ctypes.byref(cc), storage))
 
 
+def snapshot_info(path: str) -> tuple["Config", int, int]:
+    """(configuration, storage, step count) of a snapshot file."""
+    cc = _Config()
+    st = ctypes.c_int32()
+    steps = ctypes.c_int64()
+    _check(lib().cglbm_snapshot_info(path.encode(), ctypes.byref(cc), ctypes.byref(st), ctypes.byref(steps)),
+           "cglbm_snapshot_info")
+    cfg = Config(**{f.name: getattr(cc, f.name) for f in fields(Config)})
+    return cfg, int(st.value), int(steps.value)
+
+
 class Domain:
     """A library-owned domain (ping-pong moment state on the GPU)."""
 
@@ -478,6 +494,26 @@ class Domain:
 
     def write_vti(self, path: str):
         _check(lib().cglbm_write_vti(self._h, path.encode()), "cglbm_write_vti")
+
+    def save(self, path: str):
+        """Restartable snapshot of the configuration, storage, step count and full state."""
+        _check(lib().cglbm_snapshot_write(self._h, path.encode()), "cglbm_snapshot_write")
+
+    def load(self, path: str):
+        """Restore a snapshot written from a domain of the same configuration and storage."""
+        _check(lib().cglbm_snapshot_read(self._h, path.encode()), "cglbm_snapshot_read")
+
+    @classmethod
+    def from_snapshot(cls, path: str, kernel: int = KERNEL_AUTO) -> "Domain":
+        """A new domain restored from a snapshot."""
+        cfg, storage, _ = snapshot_info(path)
+        d = cls(cfg, storage=storage, kernel=kernel)
+        try:
+            d.load(path)
+        except Exception:
+            d.close()
+            raise
+        return d
 
 
 class Batch:
