@@ -95,6 +95,121 @@ def calibrate_copy(cg, mib: int):
     print(f"bench: calibration copy of {mib} MiB done", file=sys.stderr, flush=True)
 
 
+def _hip_runtime(cg):
+    """The HIP runtime the library loaded (ctypes), for events and copies around raw launches."""
+    import ctypes
+    cg.lib()
+    with open("/proc/self/maps") as f:
+        path = next(line.split()[-1] for line in f if "libamdhip64" in line)
+    hip = ctypes.CDLL(path)
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+    hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    return hip
+
+
+def measure_shard(cg, steps: int = 200, ranks: int = 8):
+    """One rank's share of the 128 x 512 x 128 Rayleigh-Taylor layer split into `ranks` z-slabs
+    (the decomposition a multi-GPU run would use; one GPU runs one shard): the z-slab update of a
+    128 x 512 x 16 slab held with two halo planes on each side (cglbm_launch_rcs_zslab), and the
+    device side of its halo exchange, staging the two boundary planes of each face (all 11 fields)
+    into send buffers and the received ones into the halo planes (cglbm_zslab_pack / unpack). The
+    exchange itself (RCCL send/receive over xGMI) is not run: one GPU."""
+    import ctypes
+    hip = _hip_runtime(cg)
+    nz_full, ny, nx = 128, 512, 128
+    own = nz_full // ranks
+    cfg = cg.Config(nx=nx, ny=ny, nz=own + 4, q=19, ybc=cg.Y_WALLS, tau=0.6, sigma=0.002, beta=0.7, gravity=1e-5,
+                    atwood=0.5)
+    plane = nx * (ny + 2)  # cells of one stored z-plane
+    with cg.Domain(cfg) as d:
+        d.init_rti(2.0, 4.0, 8, 0.2, 0)
+        d.sync()
+        s = ctypes.c_void_p(cg.lib().cglbm_domain_stream(d._h))
+        views = [d.moments(0), d.moments(1)]
+        ev = [ctypes.c_void_p() for _ in range(12)]
+        for e in ev:
+            assert hip.hipEventCreate(ctypes.byref(e)) == 0
+        off = 3 * own - 2  # the fourth rank's slab: global planes 3*own .. 4*own - 1
+        for k in range(10):  # warm-up
+            cg.launch_rcs_zslab(views[k & 1], views[(k + 1) & 1], cfg, 2, 2 + own, off, step=k + 1, stream=s.value)
+        hip.hipEventRecord(ev[0], s)
+        for k in range(steps):
+            cg.launch_rcs_zslab(views[k & 1], views[(k + 1) & 1], cfg, 2, 2 + own, off, step=k + 11, stream=s.value)
+        hip.hipEventRecord(ev[1], s)
+        # halo exchange, device side: pack both faces (13 plane-fields each) into send buffers, and
+        # unpack two buffers into the halo planes (the same buffers stand in for received ones)
+        bufs = [ctypes.c_void_p(), ctypes.c_void_p()]
+        for b in bufs:
+            assert hip.hipMalloc(ctypes.byref(b), 13 * plane * 4) == 0
+        for k in range(5):
+            cg.zslab_pack(views[0], cfg, 2, 2 + own, bufs[0].value, bufs[1].value, stream=s.value)
+        hip.hipEventRecord(ev[2], s)
+        for k in range(steps):
+            cg.zslab_pack(views[0], cfg, 2, 2 + own, bufs[0].value, bufs[1].value, stream=s.value)
+        hip.hipEventRecord(ev[3], s)
+        hip.hipEventRecord(ev[4], s)
+        for k in range(steps):
+            cg.zslab_unpack(views[0], cfg, 2, 2 + own, bufs[0].value, bufs[1].value, stream=s.value)
+        hip.hipEventRecord(ev[5], s)
+        # the schedule that hides the transfer: pack, update the interior (two planes in from each
+        # face; the transfer would run meanwhile), unpack, update the two planes next to each face
+        # (both faces in one launch)
+        hip.hipEventRecord(ev[6], s)
+        for k in range(steps):
+            A, B = views[k & 1], views[(k + 1) & 1]
+            cg.zslab_pack(A, cfg, 2, 2 + own, bufs[0].value, bufs[1].value, stream=s.value)
+            cg.launch_rcs_zslab(A, B, cfg, 4, own, off, step=k + 11, stream=s.value)
+            cg.zslab_unpack(A, cfg, 2, 2 + own, bufs[0].value, bufs[1].value, stream=s.value)
+            cg.launch_rcs_zslab_faces(A, B, cfg, 2, 2 + own, 2, off, step=k + 11, stream=s.value)
+        hip.hipEventRecord(ev[7], s)
+        # the schedule's two update parts alone
+        hip.hipEventRecord(ev[8], s)
+        for k in range(steps):
+            cg.launch_rcs_zslab(views[k & 1], views[(k + 1) & 1], cfg, 4, own, off, step=k + 11, stream=s.value)
+        hip.hipEventRecord(ev[9], s)
+        hip.hipEventRecord(ev[10], s)
+        for k in range(steps):
+            cg.launch_rcs_zslab_faces(views[k & 1], views[(k + 1) & 1], cfg, 2, 2 + own, 2, off, step=k + 11,
+                                      stream=s.value)
+        hip.hipEventRecord(ev[11], s)
+        assert hip.hipEventSynchronize(ev[11]) == 0
+        ms_inner, ms_faces = ctypes.c_float(), ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms_inner), ev[8], ev[9])
+        hip.hipEventElapsedTime(ctypes.byref(ms_faces), ev[10], ev[11])
+        ms_step, ms_pack, ms_unpack, ms_sched = ctypes.c_float(), ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms_sched), ev[6], ev[7])
+        hip.hipEventElapsedTime(ctypes.byref(ms_step), ev[0], ev[1])
+        hip.hipEventElapsedTime(ctypes.byref(ms_pack), ev[2], ev[3])
+        hip.hipEventElapsedTime(ctypes.byref(ms_unpack), ev[4], ev[5])
+        for b in bufs:
+            hip.hipFree(b)
+        for e in ev:
+            hip.hipEventDestroy(e)
+    us_step = ms_step.value * 1e3 / steps
+    cells = nx * ny * own
+    return {"domain": "RTI 128 x 512 x 128 (D3Q19, walls, forced), z-slabs", "ranks": ranks,
+            "slab": [nx, ny, own], "stored_planes": own + 4, "steps": steps,
+            "us_per_step": round(us_step, 2), "value": round(cells / (us_step * 1e-6) / 1e6, 1), "unit": "MLUPS",
+            "halo_face_bytes": 13 * plane * 4,
+            "us_pack_both_faces": round(ms_pack.value * 1e3 / steps, 2),
+            "us_unpack_both_faces": round(ms_unpack.value * 1e3 / steps, 2),
+            "us_per_step_overlap_schedule": round(ms_sched.value * 1e3 / steps, 2),
+            "us_interior_update": round(ms_inner.value * 1e3 / steps, 2),
+            "us_face_updates": round(ms_faces.value * 1e3 / steps, 2),
+            "note": "update of one rank's slab with its halo planes in place (cglbm_launch_rcs_zslab), and the "
+                    "device side of its exchange: packing both faces' 13 plane-fields into send buffers and "
+                    "unpacking two received buffers into the halo planes (cglbm_zslab_pack / unpack); "
+                    "us_per_step_overlap_schedule: pack, interior update, unpack and the two face updates in "
+                    "sequence on one stream, the device time of a rank's step whose transfer hides behind the "
+                    "interior update; the transfer between GPUs is not run"}
+
+
 def batched_workload(cg, seed: int = 0, count: int = 256):
     """Config 4: `count` independent periodic D3Q19 cubes, edges uniform in {32, 48, 64, 80, 96},
     drop radius uniform in [4, min(24, e/3)], tau uniform in [0.52, 1.0], sigma log-uniform in
@@ -455,6 +570,13 @@ def main() -> int:
         print("bench: other configurations (64^3 bubble, D3Q27 jet, RTI FP16/FP32)", file=sys.stderr, flush=True)
         configs = measure_configs(cg)
         print("bench: " + ", ".join(f"{k} {v['value']:.0f}" for k, v in configs.items()), file=sys.stderr, flush=True)
+    shard = None
+    if not args.no_configs:
+        print("bench: one z-slab shard of the 128x512x128 RTI (8-way split)", file=sys.stderr, flush=True)
+        shard = measure_shard(cg)
+        print(f"bench: shard {shard['us_per_step']:.1f} us/step, halo pack {shard['us_pack_both_faces']:.1f} us, "
+              f"unpack {shard['us_unpack_both_faces']:.1f} us, overlap schedule "
+              f"{shard['us_per_step_overlap_schedule']:.1f} us/step", file=sys.stderr, flush=True)
     serving = None
     if not args.no_serve:
         import cglbm.serving as sv
@@ -529,6 +651,7 @@ def main() -> int:
             "fp16_error_note": "relative RMS per field after 500 steps against FP64, per case kind at a mid-range draw "
                                "(profiles/precision_fp16_vs_fp64.json)"},
         "other_configs": configs,
+        "shard": shard,
     }
     print(json.dumps(line))
     return 0

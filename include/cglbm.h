@@ -144,6 +144,41 @@ int cglbm_launch_rcs_planned(const cglbm_moments *A, const cglbm_moments *B, con
 int cglbm_launch_rcs_planned_noalloc(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
                                      const cglbm_layout *layout, int64_t step, void *stream);
 
+/* One shard of a domain split into z-slabs (no counterpart in the reference, which runs on one
+ * GPU): updates target planes [z_lo, z_hi) of B from A, reading A's planes z_lo-2 .. z_hi+1. The
+ * arrays hold a slab plus two halo planes on each side (cfg->nz counts all of them), which the
+ * caller fills from the neighbouring shards before the call: plane z_lo-1 (all 11 fields) and
+ * z_lo-2 (rhor, rhob: the two-hop colour gradient) below, z_hi and z_hi+1 above. Requires
+ * 2 <= z_lo < z_hi <= nz - 2; no plane wraps. z_offset is added to a local plane index where the
+ * update depends on the global one (the jet inlet's disk and perturbation key), so the shards of a
+ * domain together give bitwise the domain's own step. x and y are treated as `cfg` says (boundary
+ * planes are written for the updated planes). z-marching kernel, uniform chunks of the slab. */
+int cglbm_launch_rcs_zslab(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
+                           const cglbm_layout *layout, int32_t z_lo, int32_t z_hi, int32_t z_offset, int64_t step,
+                           void *stream);
+
+/* The faces of a z-slab in one launch (no counterpart in the reference): target planes
+ * [z_lo, z_lo + width) and [z_hi - width, z_hi), as cglbm_launch_rcs_zslab would update them
+ * (1 <= width <= (z_hi - z_lo) / 2). With width 2 it completes a step whose interior,
+ * [z_lo + 2, z_hi - 2), was updated by cglbm_launch_rcs_zslab before the halo planes arrived: the
+ * interior reads no halo plane, so the exchange can run during it. */
+int cglbm_launch_rcs_zslab_faces(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
+                                 const cglbm_layout *layout, int32_t z_lo, int32_t z_hi, int32_t width,
+                                 int32_t z_offset, int64_t step, void *stream);
+
+/* Device side of the z-slab halo exchange (no counterpart in the reference). Each face moves 13
+ * plane-fields, nx * nyt floats each, in one contiguous buffer: the 11 fields of the plane next to
+ * the face, then rhor and rhob of the plane beyond it. cglbm_zslab_pack copies, from A, what the
+ * neighbours need: planes z_lo (fields) and z_lo+1 (rhor, rhob) into send_lo for the one below, z_hi-1
+ * and z_hi-2 into send_hi for the one above (slabs of at least 2 planes). cglbm_zslab_unpack writes
+ * a received recv_lo into the halo planes z_lo-1 and z_lo-2 and recv_hi into z_hi and z_hi+1. One
+ * launch each on `stream`; the transfer between GPUs (RCCL send/receive, peer copies) is the
+ * caller's. Same range rules as cglbm_launch_rcs_zslab. */
+int cglbm_zslab_pack(const cglbm_moments *A, const cglbm_config *cfg, const cglbm_layout *layout, int32_t z_lo,
+                     int32_t z_hi, float *send_lo, float *send_hi, void *stream);
+int cglbm_zslab_unpack(const cglbm_moments *A, const cglbm_config *cfg, const cglbm_layout *layout, int32_t z_lo,
+                       int32_t z_hi, const float *recv_lo, const float *recv_hi, void *stream);
+
 /* Boundary planes only: JetBoundaryConditions(B, Aold) (kernels.cu:102-114) or the wall copies.
  * B == Aold is allowed (main.cu:26). */
 int cglbm_launch_ybc(const cglbm_moments *B, const cglbm_moments *Aold, const cglbm_config *cfg,

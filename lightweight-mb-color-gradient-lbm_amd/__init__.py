@@ -38,7 +38,8 @@ EXPORTS = (
     "cglbm_domain_step_count", "cglbm_domain_stream", "cglbm_domain_plan",
     "cglbm_plan_create", "cglbm_plan_destroy", "cglbm_plan_reset", "cglbm_launch_rcs_planned",
     "cglbm_launch_init_bubble", "cglbm_launch_init_jet", "cglbm_launch_init_rti",
-    "cglbm_launch_rcs_planned_noalloc",
+    "cglbm_launch_rcs_planned_noalloc", "cglbm_launch_rcs_zslab", "cglbm_launch_rcs_zslab_faces", "cglbm_zslab_pack",
+    "cglbm_zslab_unpack",
     "cglbm_init_bubble", "cglbm_init_jet", "cglbm_init_rti", "cglbm_step", "cglbm_time_steps", "cglbm_sync",
     "cglbm_copy_out", "cglbm_copy_in", "cglbm_reduce", "cglbm_write_vti", "cglbm_write_vti_host",
     "cglbm_snapshot_write", "cglbm_snapshot_read", "cglbm_snapshot_info",
@@ -177,6 +178,14 @@ def lib():
         "cglbm_compact_layout": ([C, LP], ctypes.c_int),
         "cglbm_launch_rcs": ([M, M, C, LP, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p], ctypes.c_int),
         "cglbm_launch_ybc": ([M, M, C, LP, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
+        "cglbm_launch_rcs_zslab": ([M, M, C, LP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                                    ctypes.c_void_p], ctypes.c_int),
+        "cglbm_launch_rcs_zslab_faces": ([M, M, C, LP, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
+        "cglbm_zslab_pack": ([M, C, LP, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                              ctypes.c_void_p], ctypes.c_int),
+        "cglbm_zslab_unpack": ([M, C, LP, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_void_p], ctypes.c_int),
         "cglbm_launch_init_bubble": ([M, C, LP] + [ctypes.c_double] * 5 + [ctypes.c_int32, ctypes.c_void_p],
                                      ctypes.c_int),
         "cglbm_launch_init_jet": ([M, C, LP, ctypes.c_void_p], ctypes.c_int),
@@ -266,6 +275,43 @@ def launch_rcs(A: Moments, B: Moments, cfg: Config, layout: Layout | None = None
     L = layout if layout is not None else compact_layout(cfg)
     _check(lib().cglbm_launch_rcs(ctypes.byref(A), ctypes.byref(B), ctypes.byref(cc), ctypes.byref(L), step, kernel,
                                   stream), "cglbm_launch_rcs")
+
+
+def launch_rcs_zslab(A: Moments, B: Moments, cfg: Config, z_lo: int, z_hi: int, z_offset: int = 0,
+                     layout: Layout | None = None, step: int = 1, stream: int | None = None):
+    """One z-slab shard's update (cglbm_launch_rcs_zslab): target planes [z_lo, z_hi) of arrays that
+    hold the slab and two halo planes on each side; z_offset maps local planes to global ones."""
+    cc = cfg.c()
+    L = layout if layout is not None else compact_layout(cfg)
+    _check(lib().cglbm_launch_rcs_zslab(ctypes.byref(A), ctypes.byref(B), ctypes.byref(cc), ctypes.byref(L), z_lo,
+                                        z_hi, z_offset, step, stream), "cglbm_launch_rcs_zslab")
+
+
+def launch_rcs_zslab_faces(A: Moments, B: Moments, cfg: Config, z_lo: int, z_hi: int, width: int = 2,
+                           z_offset: int = 0, layout: Layout | None = None, step: int = 1, stream: int | None = None):
+    """Both faces of a z-slab in one launch (cglbm_launch_rcs_zslab_faces): target planes
+    [z_lo, z_lo + width) and [z_hi - width, z_hi)."""
+    cc = cfg.c()
+    L = layout if layout is not None else compact_layout(cfg)
+    _check(lib().cglbm_launch_rcs_zslab_faces(ctypes.byref(A), ctypes.byref(B), ctypes.byref(cc), ctypes.byref(L),
+                                              z_lo, z_hi, width, z_offset, step, stream),
+           "cglbm_launch_rcs_zslab_faces")
+
+
+def zslab_pack(A: Moments, cfg: Config, z_lo: int, z_hi: int, send_lo: int, send_hi: int,
+               layout: Layout | None = None, stream: int | None = None):
+    """Pack the 13 plane-fields each neighbour of a z-slab needs into two device buffers."""
+    L = layout if layout is not None else compact_layout(cfg)
+    _check(lib().cglbm_zslab_pack(ctypes.byref(A), ctypes.byref(cfg.c()), ctypes.byref(L), z_lo, z_hi, send_lo,
+                                  send_hi, stream), "cglbm_zslab_pack")
+
+
+def zslab_unpack(A: Moments, cfg: Config, z_lo: int, z_hi: int, recv_lo: int, recv_hi: int,
+                 layout: Layout | None = None, stream: int | None = None):
+    """Write two received device buffers into a z-slab's halo planes."""
+    L = layout if layout is not None else compact_layout(cfg)
+    _check(lib().cglbm_zslab_unpack(ctypes.byref(A), ctypes.byref(cfg.c()), ctypes.byref(L), z_lo, z_hi, recv_lo,
+                                    recv_hi, stream), "cglbm_zslab_unpack")
 
 
 class Plan:

@@ -11,6 +11,9 @@ namespace cglbm {
 hipError_t launch_gather(const StepArgs &a, int q, hipStream_t s);
 // Boundary planes of B from B and Aold (jet inlet/outlet or wall copies); no-op when periodic.
 hipError_t launch_ybc(const BcArgs &a, hipStream_t s);
+// Halo planes of a z-slab (zslab.hip): pack = copy the planes the two neighbours need from a.A into
+// lo / hi (13 * nx * nyt floats each), else write lo / hi into the slab's halo planes.
+hipError_t launch_zslab_halo(const StepArgs &a, int z_lo, int z_hi, float *lo, float *hi, bool pack, hipStream_t s);
 
 // z-marching step (any layout, fused boundary planes).
 struct MarchShape {
@@ -44,6 +47,14 @@ struct MarchArgs {
     // (kStoreF32Ptr) addresses; the buffer forms use A0/B0 and the strides.
     const float *Af[11];
     float *Bf[11];
+    // Target planes of uniform chunks: [zlo, zhi), the whole [0, nz) for a domain. A z-slab of a
+    // domain sharded along z updates its own planes between two halo planes on each side (planes
+    // zlo-2 .. zhi+1 are read, none wraps); zoff turns a local plane index into the global one
+    // where the update depends on it (the jet inlet's disk and perturbation key). zface > 0
+    // restricts the update to the zface planes next to each end of [zlo, zhi) (the faces of a slab,
+    // updated once their halos have arrived): the first half of the chunks covers the lower face,
+    // the second half the upper one.
+    int zlo, zhi, zoff, zface;
     // Optional work plan (single-domain launches): items[i] = {tile x, tile y, z0, z1}, one per
     // work-group (z0 == z1: idle). Null: uniform chunks of the launch shape's length.
     const int4 *items;

@@ -266,16 +266,17 @@ __device__ __forceinline__ void boundary_planes(const MarchArgs &a, const FieldB
             B.st(3, i0, 0.0f);
             B.st(4, i0, 0.0f);
         } else {
+            const int zg = zt + a.zoff;  // global plane of a z-slab
             const float dx = static_cast<float>(gx) - a.jet_x0;
-            const float dz = static_cast<float>(zt) - a.jet_z0;
+            const float dz = static_cast<float>(zg) - a.jet_z0;
             const float m = ((dx * dx + dz * dz) <= (a.jet_r * a.jet_r)) ? 1.0f : 0.0f;
             float U = a.jet_u, Ux = 0.0f, Uz = 0.0f;
             const bool noisy = a.jet_noise != 0.0f;
             if (noisy) {
                 const float eps = a.jet_noise;
-                const float q0 = inlet_uniform(a.seed, gx, zt, a.step, 0);
-                const float q1 = inlet_uniform(a.seed, gx, zt, a.step, 1);
-                const float q2 = inlet_uniform(a.seed, gx, zt, a.step, 2);
+                const float q0 = inlet_uniform(a.seed, gx, zg, a.step, 0);
+                const float q1 = inlet_uniform(a.seed, gx, zg, a.step, 1);
+                const float q2 = inlet_uniform(a.seed, gx, zg, a.step, 2);
                 Ux = U * (eps * (2.0f * q1 - 1.0f));
                 Uz = U * (eps * (2.0f * q2 - 1.0f));
                 U = U * (1.0f + eps * (2.0f * q0 - 1.0f));
@@ -751,9 +752,15 @@ __global__ void __launch_bounds__(block_threads<TY>()) CGLBM_MARCH_OCC
     } else {
         tix = logical % ntx;
         tiy = (logical / ntx) % nty;
-        const int chunk = logical / (ntx * nty);
-        z0 = static_cast<int>((static_cast<int64_t>(chunk) * a.nz) / nchunk);
-        z1 = static_cast<int>((static_cast<int64_t>(chunk + 1) * a.nz) / nchunk);
+        int chunk = logical / (ntx * nty), n = nchunk, lo = a.zlo, hi = a.zhi;
+        if (a.zface > 0) {  // faces of a slab: half the chunks per face
+            n = nchunk / 2;
+            if (chunk < n) hi = a.zlo + a.zface;
+            else { lo = a.zhi - a.zface; chunk -= n; }
+        }
+        const int span = hi - lo;
+        z0 = lo + static_cast<int>((static_cast<int64_t>(chunk) * span) / n);
+        z1 = lo + static_cast<int>((static_cast<int64_t>(chunk + 1) * span) / n);
     }
     march_tile<Q, YBC, TY, NBUF, FORCED, ST, REC>(a, sh, tix, tiy, z0, z1);
     if constexpr (REC) {
@@ -796,7 +803,7 @@ template <int Q, int YBC, int TY, int NBUF, int ST>
 hipError_t launch_shape(const MarchArgs &a, int L, hipStream_t s) {
     const int ntx = (a.nx + TX - 1) / TX;
     const int nty = (a.ny + TY - 1) / TY;
-    const int nchunk = (a.nz + L - 1) / L;
+    const int nchunk = a.zface > 0 ? 2 * ((a.zface + L - 1) / L) : (a.zhi - a.zlo + L - 1) / L;
     const int nblocks = a.items != nullptr ? a.nitems : ntx * nty * nchunk;
     const int grid = (nblocks + 7) / 8 * 8;
     // The body-force variant exists for bounded y (the RTI walls); a periodic forced domain takes

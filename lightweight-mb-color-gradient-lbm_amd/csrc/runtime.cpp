@@ -522,6 +522,95 @@ int cglbm_launch_rcs(const cglbm_moments *A, const cglbm_moments *B, const cglbm
     return launch_step(A, B, cfg, layout, step, kernel, nullptr, static_cast<hipStream_t>(stream));
 }
 
+int cglbm_launch_rcs_zslab(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
+                           const cglbm_layout *layout, int32_t z_lo, int32_t z_hi, int32_t z_offset, int64_t step,
+                           void *stream) {
+    if (int rc = validate_config(cfg)) return rc;
+    if (int rc = check_layout(cfg, layout)) return rc;
+    if (int rc = check_moments(A, "A")) return rc;
+    if (int rc = check_moments(B, "B")) return rc;
+    if (int rc = check_disjoint(A, B, cfg, layout)) return rc;
+    if (!(z_lo >= 2 && z_lo < z_hi && z_hi <= cfg->nz - 2))
+        return fail(CGLBM_EINVAL, "z-slab range [%d, %d) must satisfy 2 <= z_lo < z_hi <= nz - 2 (two halo planes "
+                                  "on each side; nz = %d)", z_lo, z_hi, cfg->nz);
+    const StepArgs a = make_step_args(A, B, cfg, layout, step);
+    MarchArgs ma;
+    if (!march_supports(a, cfg->q) || !march_args(a, ma))
+        return fail(CGLBM_EINVAL, "a z-slab update needs the z-marching kernel (a periodic domain with a body force "
+                                  "takes the gather kernel)");
+    ma.zlo = z_lo;
+    ma.zhi = z_hi;
+    ma.zoff = z_offset;
+    // the size-aware uniform default of a domain of the slab's own planes
+    StepArgs own = a;
+    own.nz = z_hi - z_lo;
+    const MarchShape m = march_default_shape(own, cfg->q);
+    const hipError_t e = launch_march_args(ma, cfg->q, effective_shape(cfg->q, m), static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "launch_march (z-slab)");
+    return CGLBM_OK;
+}
+
+int cglbm_launch_rcs_zslab_faces(const cglbm_moments *A, const cglbm_moments *B, const cglbm_config *cfg,
+                                 const cglbm_layout *layout, int32_t z_lo, int32_t z_hi, int32_t width,
+                                 int32_t z_offset, int64_t step, void *stream) {
+    if (int rc = validate_config(cfg)) return rc;
+    if (int rc = check_layout(cfg, layout)) return rc;
+    if (int rc = check_moments(A, "A")) return rc;
+    if (int rc = check_moments(B, "B")) return rc;
+    if (int rc = check_disjoint(A, B, cfg, layout)) return rc;
+    if (!(z_lo >= 2 && z_lo < z_hi && z_hi <= cfg->nz - 2))
+        return fail(CGLBM_EINVAL, "z-slab range [%d, %d) must satisfy 2 <= z_lo < z_hi <= nz - 2 (two halo planes "
+                                  "on each side; nz = %d)", z_lo, z_hi, cfg->nz);
+    if (width < 1 || 2 * static_cast<int64_t>(width) > z_hi - z_lo)
+        return fail(CGLBM_EINVAL, "face width %d must be at least 1 and at most half the slab (%d planes)", width,
+                    z_hi - z_lo);
+    const StepArgs a = make_step_args(A, B, cfg, layout, step);
+    MarchArgs ma;
+    if (!march_supports(a, cfg->q) || !march_args(a, ma))
+        return fail(CGLBM_EINVAL, "a z-slab update needs the z-marching kernel (a periodic domain with a body force "
+                                  "takes the gather kernel)");
+    ma.zlo = z_lo;
+    ma.zhi = z_hi;
+    ma.zoff = z_offset;
+    ma.zface = width;
+    // the size-aware uniform default of a domain of both faces' planes
+    StepArgs own = a;
+    own.nz = 2 * width;
+    const MarchShape m = march_default_shape(own, cfg->q);
+    const hipError_t e = launch_march_args(ma, cfg->q, effective_shape(cfg->q, m), static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "launch_march (z-slab faces)");
+    return CGLBM_OK;
+}
+
+static int zslab_halo(const cglbm_moments *A, const cglbm_config *cfg, const cglbm_layout *layout, int32_t z_lo,
+                      int32_t z_hi, float *lo, float *hi, bool pack, void *stream) {
+    if (int rc = validate_config(cfg)) return rc;
+    if (int rc = check_layout(cfg, layout)) return rc;
+    if (int rc = check_moments(A, "A")) return rc;
+    if (!lo || !hi) return fail(CGLBM_EINVAL, "null halo buffer");
+    if (!(z_lo >= 2 && z_lo < z_hi && z_hi <= cfg->nz - 2))
+        return fail(CGLBM_EINVAL, "z-slab range [%d, %d) must satisfy 2 <= z_lo < z_hi <= nz - 2 (two halo planes "
+                                  "on each side; nz = %d)", z_lo, z_hi, cfg->nz);
+    if (pack && z_hi - z_lo < 2) return fail(CGLBM_EINVAL, "packing needs a slab of at least 2 planes");
+    const int64_t nyt = cfg->ybc == CGLBM_Y_PERIODIC ? cfg->ny : cfg->ny + 2;
+    if (26 * static_cast<int64_t>(cfg->nx) * nyt >= (int64_t(1) << 31)) return fail(CGLBM_EINVAL, "halo planes too large");
+    const StepArgs a = make_step_args(A, A, cfg, layout, 0);
+    const hipError_t e = launch_zslab_halo(a, z_lo, z_hi, lo, hi, pack, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, pack ? "zslab pack" : "zslab unpack");
+    return CGLBM_OK;
+}
+
+int cglbm_zslab_pack(const cglbm_moments *A, const cglbm_config *cfg, const cglbm_layout *layout, int32_t z_lo,
+                     int32_t z_hi, float *send_lo, float *send_hi, void *stream) {
+    return zslab_halo(A, cfg, layout, z_lo, z_hi, send_lo, send_hi, true, stream);
+}
+
+int cglbm_zslab_unpack(const cglbm_moments *A, const cglbm_config *cfg, const cglbm_layout *layout, int32_t z_lo,
+                       int32_t z_hi, const float *recv_lo, const float *recv_hi, void *stream) {
+    return zslab_halo(A, cfg, layout, z_lo, z_hi, const_cast<float *>(recv_lo), const_cast<float *>(recv_hi), false,
+                      stream);
+}
+
 int cglbm_plan_create(const cglbm_config *cfg, cglbm_plan **out) {
     if (int rc = validate_config(cfg)) return rc;
     if (!out) return fail(CGLBM_EINVAL, "null output");

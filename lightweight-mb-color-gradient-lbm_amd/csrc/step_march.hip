@@ -94,6 +94,10 @@ bool march_args(const StepArgs &a, MarchArgs &m) {
     m.seed = a.seed;
     m.step = a.step;
     m.store = buffer ? kStoreF32 : kStoreF32Ptr;
+    m.zlo = 0;
+    m.zhi = a.nz;
+    m.zoff = 0;
+    m.zface = 0;
     m.items = nullptr;
     m.nitems = 0;
     m.wmap = nullptr;
