@@ -244,8 +244,8 @@ __device__ __forceinline__ float div_shared(float a, float b, float y) {
 // numerator gives +0 here as in IEEE division. The result differs from RN(a / b) only when the
 // remainder underflows, which needs a < 2^-100; the quotient is then below 2^-100 / 1e-8 (|F|^2 >
 // 1e-8), and this form returns a value within a few ulps of it. Either value, times the weight,
-// is below 1e-23, less than half an ulp of every B_i (at least 1/36, ulp 1.9e-9), so
-// w c2 - B_i rounds to -B_i exactly and om is the same bits.
+// is below 1e-23, less than half an ulp of every B_i (at least 1/216, the D3Q27 corner weight;
+// half an ulp 2.3e-10), so w c2 - B_i rounds to -B_i exactly and om is the same bits.
 __device__ __forceinline__ float div_markstein(float a, float b, float y) {
     float q = a * y;
     const float r = fmaf(-q, b, a);
@@ -264,9 +264,12 @@ __device__ __forceinline__ Iface iface_begin(const Mom &s) {
     return f;
 }
 
-// Q == 19 also prepares the shared reciprocals and the per-lane mask of the wave-uniform
-// interface form; D3Q27 keeps the per-direction IEEE divisions (its register budget is tighter).
-template <int Q>
+// SHARED also prepares the shared reciprocals of the interface divisions (div_markstein,
+// div_shared) and the per-lane mask of the wave-uniform interface form (D3Q19; D3Q27 with periodic
+// y, whose interface form still runs under a per-lane branch). The D3Q27 kernels with bounded y
+// keep the per-direction IEEE divisions: their register budget is tighter (the shared form spills
+// more there and measured slower on the jet).
+template <int Q, bool SHARED>
 __device__ __forceinline__ void iface_finish(Iface &f, const Mom &s, const Coeffs &c, int conservative) {
     f.af = sqrtf(f.Fx * f.Fx + f.Fy * f.Fy + f.Fz * f.Fz);
     f.af2 = f.af * f.af;
@@ -275,7 +278,7 @@ __device__ __forceinline__ void iface_finish(Iface &f, const Mom &s, const Coeff
     f.ha = 0.5f * c.Acoef * f.af;
     const float rho = s.rr + s.rb;
     f.coeff = c.beta * (s.rr * s.rb) / (rho * rho);
-    if constexpr (Q == 19) {
+    if constexpr (SHARED) {
         f.rinv_af2 = 1.0f / f.af2;
         f.rinv_af = 1.0f / f.af;
         f.omega = f.omega && f.on;
@@ -324,7 +327,7 @@ __device__ __forceinline__ void colour_gradient(PsiAt &&psi_at, float &Fx, float
 // (Omega2 and recolouring, lbm.cuh:187-208) at compile time so the per-pair work has no branches;
 // inside it the |F| <= 1e-4 and conservative cases are selects. Adding the +0 that a skipped
 // Omega2 or recolouring contributes changes no value.
-template <int Q, uint32_t PMASK, bool REST, bool FORCED, bool IFACE, typename Sink>
+template <int Q, uint32_t PMASK, bool REST, bool FORCED, bool IFACE, bool SHARED, typename Sink>
 __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, const Iface &fi, Sink &&sink) {
     constexpr bool forced = FORCED;
     const float rho = s.rr + s.rb;
@@ -383,7 +386,7 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
                 gi = feqi + c.oms * gneqi;
                 go = feqo + c.oms * gneqo;
             }
-            if constexpr (IFACE && Q != 19) {
+            if constexpr (IFACE && !SHARED) {
                 const float Fc = cdot<I>(fi.Fx, fi.Fy, fi.Fz);
                 const float c2q = (Fc * Fc) / fi.af2;
                 const float c2 = fi.small ? 0.0f : c2q;
@@ -434,7 +437,7 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
 // emit_regular hands to the sink. sink(integral_constant<I>, integral_constant<bool, OMASK bit>,
 // fr, fb); the member the lane took is O exactly when that bit differs from `flip`. No body-force
 // form (the forced kernels emit both members).
-template <int Q, uint32_t PMASK, uint32_t OMASK, bool IFACE, typename Sink>
+template <int Q, uint32_t PMASK, uint32_t OMASK, bool IFACE, bool SHARED, typename Sink>
 __device__ __forceinline__ void emit_one(const Mom &s, const Coeffs &c, const Iface &fi, uint32_t flip, Sink &&sink) {
     const float rho = s.rr + s.rb;
     const float inv = 1.0f / (s.rr + s.rb);
@@ -464,7 +467,7 @@ __device__ __forceinline__ void emit_one(const Mom &s, const Coeffs &c, const If
             float g = feq + c.oms * (w * (An2 + An3));
             const auto IC = std::integral_constant<int, I>{};
             const auto OC = std::integral_constant<bool, OM>{};
-            if constexpr (IFACE && Q != 19) {
+            if constexpr (IFACE && !SHARED) {
                 const float Fc = cdot<I>(fi.Fx, fi.Fy, fi.Fz);
                 const float c2q = (Fc * Fc) / fi.af2;
                 const float c2 = fi.small ? 0.0f : c2q;
