@@ -20,7 +20,9 @@ def _cases(oracle):
                         sigma=0.005, beta=0.7, jet_radius=4.0, jet_velocity=0.05, jet_noise=0.01, seed=5)
     rti = oracle.Params(nx=16, ny=24, nz=24, q=19, ybc=oracle.Y_WALLS, tau=0.6, sigma=0.002, beta=0.7, gravity=1e-4,
                         atwood=0.5)
+    odd = oracle.Params(nx=18, ny=13, nz=24, q=27, tau=0.7, sigma=0.01, beta=0.7)  # rows of 18: scalar halo copies
     return {
+        "drop_q27_nx18": (odd, lambda: oracle.init_bubble(odd, radius=5.0, width=4.0)),
         "drop_q19": (drop, lambda: oracle.init_bubble(drop, radius=6.0, width=4.0)),
         "jet_q27_noise": (jet, lambda: oracle.init_jet(jet)),
         "rti_q19_forced": (rti, lambda: oracle.init_rti(rti, 2.0, 4.0, 8, 0.2, 0)),
@@ -100,7 +102,7 @@ def test_zslab_refuses_ranges_without_halos():
 
 
 @pytest.mark.parametrize("schedule", ["whole", "overlapped"])
-@pytest.mark.parametrize("label", ["drop_q19", "jet_q27_noise", "rti_q19_forced"])
+@pytest.mark.parametrize("label", ["drop_q19", "jet_q27_noise", "rti_q19_forced", "drop_q27_nx18"])
 def test_zslab_exchange_through_pack_and_unpack(oracle, label, schedule):
     """The same decomposition with the halos moved by the library's own exchange kernels: every
     shard holds only its planes (halo planes zero), packs what its neighbours need, receives their
