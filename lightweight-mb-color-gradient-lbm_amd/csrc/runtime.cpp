@@ -310,6 +310,16 @@ hipError_t launch_default(const MarchArgs &ma, const cglbm_config &c, MarchPlanS
     if (first.L == 0) cand[n++] = dflt;
     const MarchShape narrow{4, c.nz < 32 ? c.nz : 32, 2};
     if (c.q == 19 && !(dflt.ty == narrow.ty && dflt.L == narrow.L)) cand[n++] = narrow;
+    // 16-row tiles with the longest uniform chunks that still give three rounds of work-groups:
+    // fewer chunk edges than the default's shorter chunks, rounds enough for the dispatcher to
+    // even out heavy columns (the autotuner's pick on the 128 x 512 x 128 Rayleigh-Taylor layer)
+    {
+        const int64_t cols = static_cast<int64_t>(ceil_div(c.nx, kTX)) * ceil_div(c.ny, 16);
+        const int64_t nch = (3 * static_cast<int64_t>(plan->ncu) + cols - 1) / cols;
+        const int L3 = nch >= 2 ? ceil_div(c.nz, static_cast<int>(nch < c.nz ? nch : c.nz)) : 0;
+        const MarchShape three{16, L3, c.q == 27 ? 1 : 2};
+        if (L3 >= 8 && !(dflt.ty == three.ty && dflt.L == three.L) && n < kTrialMaxCand) cand[n++] = three;
+    }
     auto run = [&](const MarchShape &m) { return launch_march_planned(ma, c, m, plan, dflt.L, s); };
     hipError_t e = hipSuccess;
     switch (plan->trial) {
