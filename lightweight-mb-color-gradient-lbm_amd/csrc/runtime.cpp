@@ -310,15 +310,21 @@ hipError_t launch_default(const MarchArgs &ma, const cglbm_config &c, MarchPlanS
     if (first.L == 0) cand[n++] = dflt;
     const MarchShape narrow{4, c.nz < 32 ? c.nz : 32, 2};
     if (c.q == 19 && !(dflt.ty == narrow.ty && dflt.L == narrow.L)) cand[n++] = narrow;
-    // 16-row tiles with the longest uniform chunks that still give three rounds of work-groups:
-    // fewer chunk edges than the default's shorter chunks, rounds enough for the dispatcher to
-    // even out heavy columns (the autotuner's pick on the 128 x 512 x 128 Rayleigh-Taylor layer)
-    {
-        const int64_t cols = static_cast<int64_t>(ceil_div(c.nx, kTX)) * ceil_div(c.ny, 16);
-        const int64_t nch = (3 * static_cast<int64_t>(plan->ncu) + cols - 1) / cols;
-        const int L3 = nch >= 2 ? ceil_div(c.nz, static_cast<int>(nch < c.nz ? nch : c.nz)) : 0;
-        const MarchShape three{16, L3, c.q == 27 ? 1 : 2};
-        if (L3 >= 8 && !(dflt.ty == three.ty && dflt.L == three.L) && n < kTrialMaxCand) cand[n++] = three;
+    // 16-row tiles with the longest uniform chunks that still give one round of work-groups (no
+    // re-plans; the autotuner's pick on the 64^3 bubble) and three rounds (fewer chunk edges than
+    // the default's shorter chunks, rounds enough for the dispatcher to even out heavy columns;
+    // the autotuner's pick on the 128 x 512 x 128 Rayleigh-Taylor layer), at least 8 planes
+    auto taken = [&](const MarchShape &m) {
+        for (int k = 0; k < n; ++k)
+            if (cand[k].ty == m.ty && cand[k].L == m.L) return true;
+        return dflt.ty == m.ty && dflt.L == m.L;
+    };
+    const int64_t cols = static_cast<int64_t>(ceil_div(c.nx, kTX)) * ceil_div(c.ny, 16);
+    for (const int rounds : {1, 3}) {
+        const int64_t nch = (rounds * static_cast<int64_t>(plan->ncu) + cols - 1) / cols;
+        const int L = ceil_div(c.nz, static_cast<int>(nch < c.nz ? nch : c.nz));
+        const MarchShape m{16, L, c.q == 27 ? 1 : 2};
+        if ((rounds == 1 || L >= 8) && n < kTrialMaxCand && !taken(m)) cand[n++] = m;
     }
     auto run = [&](const MarchShape &m) { return launch_march_planned(ma, c, m, plan, dflt.L, s); };
     hipError_t e = hipSuccess;

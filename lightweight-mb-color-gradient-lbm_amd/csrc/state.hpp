@@ -22,7 +22,7 @@ BcArgs make_bc_args(const cglbm_moments *B, const cglbm_moments *Aold, const cgl
 // Balanced z-chunk plan of a library-owned domain (plan.hip). A marching shape with L == 0 asks
 // for it: the step after every `period` steps (and the first) records the interface map and is
 // followed by the planner; the steps in between take their chunks from `items`.
-inline constexpr int kTrialMaxCand = 4;
+inline constexpr int kTrialMaxCand = 5;
 struct MarchPlanState {
     int4 *items = nullptr;    // cap work items
     uint8_t *wmap = nullptr;  // interface map of the last recording step (plan_wmap_bytes)
