@@ -541,7 +541,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the update of one domain "
-              f"runs on one GPU; multi-GPU sharding is a design note, not a built path)", file=sys.stderr)
+              f"runs on one GPU; the `shard` leg times one rank's share of a domain split into z-slabs, "
+              f"and the exchange between GPUs is not run)", file=sys.stderr)
         return 2
     cg = _load()
     if args.valu_split:
