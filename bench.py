@@ -113,22 +113,37 @@ def _hip_runtime(cg):
     return hip
 
 
-def measure_shard(cg, steps: int = 200, ranks: int = 8):
-    """One rank's share of the 128 x 512 x 128 Rayleigh-Taylor layer split into `ranks` z-slabs
-    (the decomposition a multi-GPU run would use; one GPU runs one shard): the z-slab update of a
-    128 x 512 x 16 slab held with two halo planes on each side (cglbm_launch_rcs_zslab), and the
-    device side of its halo exchange, staging the two boundary planes of each face (all 11 fields)
-    into send buffers and the received ones into the halo planes (cglbm_zslab_pack / unpack). The
+def measure_shard(cg, case: str = "rti", steps: int = 200, ranks: int = 8):
+    """One rank's share of a domain split into `ranks` z-slabs (the decomposition a multi-GPU run
+    would use; one GPU runs one shard), the fourth rank's slab held with two halo planes on each
+    side: its update (cglbm_launch_rcs_zslab) and the device side of its halo exchange, packing
+    the 13 plane-fields of each face into send buffers and unpacking received ones into the halo
+    planes (cglbm_zslab_pack / unpack). case "rti": the 128 x 512 x 128 Rayleigh-Taylor layer
+    (walls and gravity along y, so z-slabs of 16 planes with 128 x 514 faces); "periodic": a
+    periodic 128 x 128 x 512 column with a drop in each slab (64-plane slabs, 128 x 128 faces). The
     exchange itself (RCCL send/receive over xGMI) is not run: one GPU."""
     import ctypes
     hip = _hip_runtime(cg)
-    nz_full, ny, nx = 128, 512, 128
-    own = nz_full // ranks
-    cfg = cg.Config(nx=nx, ny=ny, nz=own + 4, q=19, ybc=cg.Y_WALLS, tau=0.6, sigma=0.002, beta=0.7, gravity=1e-5,
-                    atwood=0.5)
-    plane = nx * (ny + 2)  # cells of one stored z-plane
+    if case == "rti":
+        nz_full, ny, nx = 128, 512, 128
+        own = nz_full // ranks
+        cfg = cg.Config(nx=nx, ny=ny, nz=own + 4, q=19, ybc=cg.Y_WALLS, tau=0.6, sigma=0.002, beta=0.7,
+                        gravity=1e-5, atwood=0.5)
+        plane = nx * (ny + 2)  # cells of one stored z-plane
+        label = "RTI 128 x 512 x 128 (D3Q19, walls, forced), z-slabs"
+    elif case == "periodic":
+        nz_full, ny, nx = 512, 128, 128
+        own = nz_full // ranks
+        cfg = cg.Config(nx=nx, ny=ny, nz=own + 4, q=19, tau=0.8, sigma=0.01, beta=0.7)
+        plane = nx * ny
+        label = "periodic 128 x 128 x 512 column (D3Q19, a drop of radius 24 per slab), z-slabs"
+    else:
+        raise ValueError(case)
     with cg.Domain(cfg) as d:
-        d.init_rti(2.0, 4.0, 8, 0.2, 0)
+        if case == "rti":
+            d.init_rti(2.0, 4.0, 8, 0.2, 0)
+        else:
+            d.init_bubble(radius=24.0, width=4.0)
         d.sync()
         s = ctypes.c_void_p(cg.lib().cglbm_domain_stream(d._h))
         views = [d.moments(0), d.moments(1)]
@@ -193,7 +208,7 @@ def measure_shard(cg, steps: int = 200, ranks: int = 8):
             hip.hipEventDestroy(e)
     us_step = ms_step.value * 1e3 / steps
     cells = nx * ny * own
-    return {"domain": "RTI 128 x 512 x 128 (D3Q19, walls, forced), z-slabs", "ranks": ranks,
+    return {"domain": label, "ranks": ranks,
             "slab": [nx, ny, own], "stored_planes": own + 4, "steps": steps,
             "us_per_step": round(us_step, 2), "value": round(cells / (us_step * 1e-6) / 1e6, 1), "unit": "MLUPS",
             "halo_face_bytes": 13 * plane * 4,
@@ -573,11 +588,13 @@ def main() -> int:
         print("bench: " + ", ".join(f"{k} {v['value']:.0f}" for k, v in configs.items()), file=sys.stderr, flush=True)
     shard = None
     if not args.no_configs:
-        print("bench: one z-slab shard of the 128x512x128 RTI (8-way split)", file=sys.stderr, flush=True)
-        shard = measure_shard(cg)
-        print(f"bench: shard {shard['us_per_step']:.1f} us/step, halo pack {shard['us_pack_both_faces']:.1f} us, "
-              f"unpack {shard['us_unpack_both_faces']:.1f} us, overlap schedule "
-              f"{shard['us_per_step_overlap_schedule']:.1f} us/step", file=sys.stderr, flush=True)
+        shard = {}
+        for key, case in (("rti_128x512x128", "rti"), ("periodic_128x128x512", "periodic")):
+            print(f"bench: one z-slab shard of the {key} domain (8-way split)", file=sys.stderr, flush=True)
+            r = shard[key] = measure_shard(cg, case)
+            print(f"bench: shard {r['us_per_step']:.1f} us/step, halo pack {r['us_pack_both_faces']:.1f} us, "
+                  f"unpack {r['us_unpack_both_faces']:.1f} us, overlap schedule "
+                  f"{r['us_per_step_overlap_schedule']:.1f} us/step", file=sys.stderr, flush=True)
     serving = None
     if not args.no_serve:
         import cglbm.serving as sv
