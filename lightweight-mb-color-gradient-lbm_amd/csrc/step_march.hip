@@ -16,13 +16,17 @@ int env_int(const char *name, int lo, int hi) {
 }  // namespace
 
 MarchShape march_default_shape(const StepArgs &a, int q) {
-    // Untuned default, size-aware: the first shape, in order of per-cell efficiency measured by the
-    // tile sweeps on MI355X (both stencils), that gives every CU a work-group (256); if none does
-    // (small domains), the one with the most work-groups. Two exchange buffers (one barrier per
+    // Untuned default, size-aware: the first shape of the list that gives every CU a work-group
+    // (256); if none does (small domains), the one with the most work-groups. 16-row tiles with
+    // shorter chunks come before 4-row tiles: on domains of 0.26 to 1.6 M cells, where 16-row
+    // tiles with 16-plane chunks leave CUs idle, 16 rows x 8 planes ran 20-27% faster than 4 rows
+    // x 16 planes in D3Q19 (drops in 128 x 128 x 48/64/96, 128 x 64 x 128, 64 x 128 x 128, 96^3)
+    // and twice as fast in D3Q27 (128 x 128 x 64), and the autotuner picks 16 rows x 2 planes on
+    // the 64^3 drop. Two exchange buffers (one barrier per
     // plane; D3Q27 16-row tiles keep one, effective_shape): measured as fast or faster than one in
     // every case swept (128 x 512 x 128 Rayleigh-Taylor, 16 rows x 16 planes: 458 against 508 us).
     // CGLBM_MARCH_TY / CGLBM_MARCH_L / CGLBM_MARCH_NBUF override the choice for tuning sweeps.
-    static constexpr int kCand[8][2] = {{16, 16}, {4, 16}, {16, 8}, {4, 8}, {16, 4}, {4, 4}, {4, 2}, {4, 1}};
+    static constexpr int kCand[9][2] = {{16, 16}, {16, 8}, {16, 4}, {16, 2}, {4, 16}, {4, 8}, {4, 4}, {4, 2}, {4, 1}};
     constexpr int64_t kFill = 256;
     (void)q;
     const auto &cand = kCand;
