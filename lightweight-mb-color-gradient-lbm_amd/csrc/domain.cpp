@@ -96,7 +96,8 @@ int apply_ybc_views(cglbm_domain *d, const cglbm_moments &m) {
 int upload_init(cglbm_domain *d, const std::vector<float> &host, bool bc) {
     const int64_t n = stored_cells(d->cfg);
     d->plan.ty = 0;  // a new state: the next planned step records its interface map afresh
-    d->plan.trial = kTrialNone;  // and the default shapes race again
+    d->plan.trial = kTrialNone;  // and the default shapes race again, soon again after that
+    d->plan.trial_period = kTrialFirstPeriod;
     Staging st;
     const cglbm_moments *m = &d->view[d->cur];
     if (d->storage == CGLBM_STORE_F16) {
@@ -554,6 +555,7 @@ int cglbm_copy_in(cglbm_domain *dom, uint32_t field_mask, const float *host) {
     const bool f16 = dom->storage == CGLBM_STORE_F16;
     dom->plan.ty = 0;
     dom->plan.trial = kTrialNone;
+    dom->plan.trial_period = kTrialFirstPeriod;
     Staging st;
     if (f16 && (field_mask & ~0x3u))
         if (int rc = make_staging(dom, st, 1)) return rc;

@@ -291,10 +291,10 @@ hipError_t launch_march_planned(MarchArgs ma, const cglbm_config &c, MarchShape 
 }
 // The step of a domain left to the default shape (no tile set, no autotuned record). A short race
 // (MarchPlanState::trial) picks among planned chunks where untuned_shape plans them, the size-aware
-// uniform default, and, for D3Q19, 4-row tiles with 32-plane chunks: each runs kTrialSteps steps
-// timed with events on the domain's stream, the host waits for the last of them (a few
-// milliseconds, once per race), the fastest is kept, and the race runs again every kTrialPeriod
-// steps. Planned chunks lose where the heavy column tiles are heavy along their whole
+// uniform default, for D3Q19 4-row tiles with 32-plane chunks, and 16-row tiles with one and three
+// rounds of uniform chunks: each runs kTrialSteps steps timed with events on the domain's stream,
+// the host waits for the last of them (a few milliseconds, once per race), the fastest is kept,
+// and the race runs again after 128, 512 and then every 2048 steps (MarchPlanState::trial_period). Planned chunks lose where the heavy column tiles are heavy along their whole
 // length and one round of work-groups cannot even them out (a flat Rayleigh-Taylor layer across x
 // and z); states too large for the 256 MB last-level cache run faster with the smaller 4-row
 // work-groups, three of which share a CU (the 128 x 512 x 128 Rayleigh-Taylor case in FP32). The
@@ -330,7 +330,8 @@ hipError_t launch_default(const MarchArgs &ma, const cglbm_config &c, MarchPlanS
     hipError_t e = hipSuccess;
     switch (plan->trial) {
     case kTrialKeep:
-        if (++plan->trial_since < kTrialPeriod) return run(plan->keep);
+        if (++plan->trial_since < plan->trial_period) return run(plan->keep);
+        plan->trial_period = plan->trial_period * 4 < kTrialPeriod ? plan->trial_period * 4 : kTrialPeriod;
         plan->trial = kTrialNone;  // race again: the interface has moved
         if (plan->keep.L != 0) plan->ty = 0;  // planned chunks race on a fresh plan
         return run(plan->keep);

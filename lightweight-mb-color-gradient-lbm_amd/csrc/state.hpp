@@ -23,6 +23,12 @@ BcArgs make_bc_args(const cglbm_moments *B, const cglbm_moments *Aold, const cgl
 // for it: the step after every `period` steps (and the first) records the interface map and is
 // followed by the planner; the steps in between take their chunks from `items`.
 inline constexpr int kTrialMaxCand = 5;
+inline constexpr int kTrialSteps = 3;
+// A decision is kept kTrialFirstPeriod steps after a state is set, then four times as long after
+// each race, up to kTrialPeriod: a young state (a jet that has just started, a drop still
+// relaxing) changes fastest, and a race at its first steps can pick a shape that loses later.
+inline constexpr int64_t kTrialFirstPeriod = 128;
+inline constexpr int64_t kTrialPeriod = 2048;
 struct MarchPlanState {
     int4 *items = nullptr;    // cap work items
     uint8_t *wmap = nullptr;  // interface map of the last recording step (plan_wmap_bytes)
@@ -35,7 +41,7 @@ struct MarchPlanState {
     // Shape race of a domain left to the default shape: planned chunks (where they apply, once the
     // first plan and its measured correction are made) and the uniform candidates each run
     // kTrialSteps steps between two events on the domain's stream; the host then waits for the last
-    // event and keeps the fastest shape for kTrialPeriod steps, after which the race runs again
+    // event and keeps the fastest shape for trial_period steps, after which the race runs again
     // (the interface moves), or until the state is replaced.
     // Results are bitwise the same for every shape. No events (caller plans): no race.
     hipEvent_t tev[2 * kTrialMaxCand] = {};
@@ -43,6 +49,7 @@ struct MarchPlanState {
     int trial_cand = 0;       // candidate being timed
     int trial_left = 0;       // steps left in its timed phase
     int64_t trial_since = 0;  // steps since the last decision
+    int64_t trial_period = kTrialFirstPeriod;  // steps the current decision is kept
     MarchShape keep{0, 0, 0}; // shape kept by the last race (ty == 0: none yet)
 };
 enum : int {
@@ -50,8 +57,6 @@ enum : int {
     kTrialTiming = 1,  // timing candidate trial_cand
     kTrialKeep = 2,    // `keep` decided
 };
-inline constexpr int kTrialSteps = 3;
-inline constexpr int64_t kTrialPeriod = 2048;
 // Work items of a planned step for tiles of `ty` rows: one per CU, rounded up to a multiple of
 // the CU count so every column tile gets at least one.
 int plan_items(const cglbm_config &c, int ty, int ncu);
