@@ -344,7 +344,7 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
         float gneq = weight<Q>(0) * A2n;
         float g;
         if constexpr (forced) {
-            const float Fi = guo_full<Q, 0>(0.0f, Fby, 0.0f, s.ux, s.uy, s.uz);
+            const float Fi = guo_y<Q, 0>(Fby, s.ux, s.uy, s.uz);
             gneq = gneq - 0.5f * Fi;
             g = feq0 + c.oms * gneq + c.one_minus_half_omega * Fi;
         } else {
@@ -376,8 +376,8 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
             float gneqo = w * (An2 - An3);
             float gi, go;
             if constexpr (forced) {
-                const float Fii = guo_full<Q, I>(0.0f, Fby, 0.0f, s.ux, s.uy, s.uz);
-                const float Fio = guo_full<Q, O>(0.0f, Fby, 0.0f, s.ux, s.uy, s.uz);
+                const float Fii = guo_y<Q, I>(Fby, s.ux, s.uy, s.uz);
+                const float Fio = guo_y<Q, O>(Fby, s.ux, s.uy, s.uz);
                 gneqi = gneqi - 0.5f * Fii;
                 gneqo = gneqo - 0.5f * Fio;
                 gi = feqi + c.oms * gneqi + c.one_minus_half_omega * Fii;

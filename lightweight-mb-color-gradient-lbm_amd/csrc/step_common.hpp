@@ -79,7 +79,7 @@ __device__ __forceinline__ float fneqr_full(float Pxx, float Pxy, float Pyy, flo
 
 // guo_force (pdf.cuh:94-136), dormant in the reference; used by the body-force option.
 template <int Q, int I>
-__device__ __forceinline__ float guo_full(float Fx, float Fy, float Fz, float ux, float uy, float uz) {
+__host__ __device__ __forceinline__ float guo_full(float Fx, float Fy, float Fz, float ux, float uy, float uz) {
     constexpr float wi = weight<Q>(I);
     constexpr float cx = static_cast<float>(kCx[I]), cy = static_cast<float>(kCy[I]), cz = static_cast<float>(kCz[I]);
     const float h = 0.5f;
@@ -91,6 +91,42 @@ __device__ __forceinline__ float guo_full(float Fx, float Fy, float Fz, float ux
                       (Fy * uy * uz + h * Fz * uy * uy) * hyyz(I) + (Fz * uy * uz + h * Fy * uz * uz) * hyzz(I) +
                       (Fx * uy * uz + Fy * ux * uz + Fz * ux * uy) * hxyz(I)) * kInvCs6;
     return wi * (A1 + A2 + A3);
+}
+
+// guo_full(0, Fy, 0, ux, uy, uz), the only force the build applies (gravity along y), without the
+// terms that are a product with the zero force components or with a zero Hermite coefficient.
+// Those terms are +0 or -0 for finite u; adding one to a partial sum leaves every nonzero sum as
+// it is, so each surviving term enters the same sums in the same order and with the same
+// rounding as in guo_full. The two can differ only in the sign of a zero result, which the step
+// then adds to a nonzero equilibrium (feq > 0 for rho > 0), so the populations are the same
+// bits. Without fast-math the compiler may not drop 0 * x itself (x could be inf or NaN).
+template <int Q, int I>
+__host__ __device__ __forceinline__ float guo_y(float Fy, float ux, float uy, float uz) {
+    constexpr float wi = weight<Q>(I);
+    constexpr float cy = static_cast<float>(kCy[I]);
+    const float h = 0.5f;
+    // A2 = ((Fy uy hyy + Fy ux hxy) + Fy uz hyz) / cs^4
+    float A2 = (Fy * uy) * hyy(I);
+    if constexpr (hxy(I) != 0.0f) A2 = A2 + (Fy * ux) * hxy(I);
+    if constexpr (hyz(I) != 0.0f) A2 = A2 + (Fy * uz) * hyz(I);
+    A2 = A2 * kInvCs4;
+    // A3 terms in guo_full's order: xxy, xyy, yyz, yzz, xyz
+    float A3 = 0.0f;
+    bool any = false;
+    auto add = [&](float t) {
+        A3 = any ? A3 + t : t;
+        any = true;
+    };
+    if constexpr (hxxy(I) != 0.0f) add(((h * Fy) * ux * ux) * hxxy(I));
+    if constexpr (hxyy(I) != 0.0f) add((Fy * ux * uy) * hxyy(I));
+    if constexpr (hyyz(I) != 0.0f) add((Fy * uy * uz) * hyyz(I));
+    if constexpr (hyzz(I) != 0.0f) add(((h * Fy) * uz * uz) * hyzz(I));
+    if constexpr (hxyz(I) != 0.0f) add((Fy * ux * uz) * hxyz(I));
+    float sum;
+    if constexpr (cy != 0.0f) sum = ((Fy * cy) * kInvCs2) + A2;
+    else sum = A2;
+    if (any) sum = sum + A3 * kInvCs6;
+    return wi * sum;
 }
 
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
