@@ -376,8 +376,8 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
             float gneqo = w * (An2 - An3);
             float gi, go;
             if constexpr (forced) {
-                const float Fii = guo_y<Q, I>(Fby, s.ux, s.uy, s.uz);
-                const float Fio = guo_y<Q, O>(Fby, s.ux, s.uy, s.uz);
+                float Fii, Fio;
+                guo_y_pair<Q, I>(Fby, s.ux, s.uy, s.uz, Fii, Fio);
                 gneqi = gneqi - 0.5f * Fii;
                 gneqo = gneqo - 0.5f * Fio;
                 gi = feqi + c.oms * gneqi + c.one_minus_half_omega * Fii;

@@ -129,6 +129,46 @@ __host__ __device__ __forceinline__ float guo_y(float Fy, float ux, float uy, fl
     return wi * sum;
 }
 
+// guo_y of direction I and of its opposite I + 1 from one set of products: the opposite direction
+// has the same weight and second-order Hermite values and negated first- and third-order ones
+// (binary32-exact: -a - (-b) = -(a - b)), so its first- and third-order parts are the negated
+// parts of direction I. Each result has guo_y's bits, up to the sign of an all-zero result (Fy = 0),
+// which, as above, changes no population.
+template <int Q, int I>
+__host__ __device__ __forceinline__ void guo_y_pair(float Fy, float ux, float uy, float uz, float &Fi, float &Fo) {
+    constexpr float wi = weight<Q>(I);
+    constexpr float cy = static_cast<float>(kCy[I]);
+    const float h = 0.5f;
+    float A2 = (Fy * uy) * hyy(I);
+    if constexpr (hxy(I) != 0.0f) A2 = A2 + (Fy * ux) * hxy(I);
+    if constexpr (hyz(I) != 0.0f) A2 = A2 + (Fy * uz) * hyz(I);
+    A2 = A2 * kInvCs4;
+    float A3 = 0.0f;
+    bool any = false;
+    auto add = [&](float t) {
+        A3 = any ? A3 + t : t;
+        any = true;
+    };
+    if constexpr (hxxy(I) != 0.0f) add(((h * Fy) * ux * ux) * hxxy(I));
+    if constexpr (hxyy(I) != 0.0f) add((Fy * ux * uy) * hxyy(I));
+    if constexpr (hyyz(I) != 0.0f) add((Fy * uy * uz) * hyyz(I));
+    if constexpr (hyzz(I) != 0.0f) add(((h * Fy) * uz * uz) * hyzz(I));
+    if constexpr (hxyz(I) != 0.0f) add((Fy * ux * uz) * hxyz(I));
+    float si = A2, so = A2;
+    if constexpr (cy != 0.0f) {
+        const float A1 = (Fy * cy) * kInvCs2;
+        si = A1 + A2;
+        so = -A1 + A2;
+    }
+    if (any) {
+        const float A3s = A3 * kInvCs6;
+        si = si + A3s;
+        so = so + -A3s;
+    }
+    Fi = wi * si;
+    Fo = wi * so;
+}
+
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 }  // namespace cglbm

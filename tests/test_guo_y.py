@@ -2,8 +2,9 @@
 against guo_full(0, Fy, 0, u) (the general Guo term of the gather kernel and of the oracle): bitwise
 equal for every direction of both stencils on random inputs (zeros of both signs, magnitudes
 2^-27..2^3 of either sign), apart from the sign of an all-zero result, which the step adds to a
-positive equilibrium. Compiled for the host with hipcc (the functions are host and device code),
-no contraction."""
+positive equilibrium. guo_y_pair (the pair form of the own sources) must give guo_y of direction I
+and of its opposite I + 1 with the same tolerance for zero signs. Compiled for the host with hipcc
+(the functions are host and device code), no contraction."""
 import os
 import shutil
 import subprocess
@@ -40,6 +41,14 @@ template <int Q, int I> long check(long n) {
         const float b = guo_y<Q, I>(v[0], v[1], v[2], v[3]);
         ++total;
         if (bf(a) != bf(b) && !(a == 0.0f && b == 0.0f)) ++bad;
+        if constexpr (I % 2 == 1) {
+            float fi, fo;
+            guo_y_pair<Q, I>(v[0], v[1], v[2], v[3], fi, fo);
+            const float o = guo_y<Q, I + 1>(v[0], v[1], v[2], v[3]);
+            total += 2;
+            if (bf(fi) != bf(b) && !(fi == 0.0f && b == 0.0f)) ++bad;
+            if (bf(fo) != bf(o) && !(fo == 0.0f && o == 0.0f)) ++bad;
+        }
     }
     return bad;
 }
@@ -67,5 +76,5 @@ def test_guo_y_equals_guo_full_with_zero_x_and_z_force():
                        check=True, capture_output=True)
         total, bad = (int(v) for v in subprocess.run([exe], capture_output=True, text=True,
                                                      check=True).stdout.split())
-    assert total == 46 * 100000
+    assert total == (46 + 2 * 22) * 100000
     assert bad == 0
