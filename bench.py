@@ -544,7 +544,7 @@ def main() -> int:
     ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--no-serve", action="store_true", help="skip the Poisson serving measurement")
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations (bubble64, jet, RTI)")
-    ap.add_argument("--serve-rates", default="6,15",
+    ap.add_argument("--serve-rates", default="6,22",
                     help="serving arrival rates (per second), comma-separated: one Poisson window each")
     ap.add_argument("--serve-durations", default="40,30", help="arrival window of each rate (s), comma-separated")
     ap.add_argument("--tile19", default=None, help="fixed D3Q19 tile 'rows,chunk,buffers' instead of autotuning")
