@@ -539,13 +539,8 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
 #else
         const Mom cur = Abuf.mom(lcell(pz));
 #endif
-        // the adjacent interior cell of a ghost source: loaded by every lane of a wave that holds a
-        // ghost source (unmasked, as above); read only by ghost sources
-        Mom curF;
-        if constexpr (YBC != kYPeriodic) {
-            if (__builtin_amdgcn_ballot_w64(sghost) != 0)
-                curF = Abuf.mom(sghost ? cell(gx, frow, pz) : lcell(pz));
-        }
+        // the adjacent interior cell of a ghost source is loaded in the ghost branch, so its 11
+        // values are live only there
 
         // ---- sources ----------------------------------------------------------------------
         [[maybe_unused]] bool ion = false;  // this lane's source took the interface form (REC)
@@ -577,7 +572,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
                 constexpr uint32_t M = decltype(MASKC)::value;
                 constexpr bool R = decltype(RESTC)::value;
                 if (YBC != kYPeriodic && sghost) {
-                    emit_ghost<Q, M, R>(cur, curF, c, sink);
+                    emit_ghost<Q, M, R>(cur, Abuf.mom(cell(gx, frow, pz)), c, sink);
                 } else {
                     Iface f = iface_begin(cur);
                     // D3Q19: wave-uniform choice; a wave with any interface source runs the
@@ -616,7 +611,7 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
                 constexpr uint32_t M = decltype(MASKC)::value;
                 constexpr uint32_t OM = decltype(OMASKC)::value;
                 if (YBC != kYPeriodic && sghost) {
-                    emit_ghost<Q, M, false>(cur, curF, c, sink3);
+                    emit_ghost<Q, M, false>(cur, Abuf.mom(cell(gx, frow, pz)), c, sink3);
                 } else {
                     Iface f = iface_begin(cur);
                     const bool iface = Q == 19 ? (__builtin_amdgcn_ballot_w64(f.on) != 0) : f.on;
