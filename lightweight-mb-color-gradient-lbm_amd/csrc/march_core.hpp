@@ -264,12 +264,10 @@ __device__ __forceinline__ Iface iface_begin(const Mom &s) {
     return f;
 }
 
-// SHARED also prepares the shared reciprocals of the interface divisions (div_markstein,
-// div_shared) and the per-lane mask of the wave-uniform interface form (D3Q19; D3Q27 with periodic
-// y, whose interface form still runs under a per-lane branch). The D3Q27 kernels with bounded y
-// keep the per-direction IEEE divisions: their register budget is tighter (the shared form spills
-// more there and measured slower on the jet).
-template <int Q, bool SHARED>
+// Also prepares the shared reciprocals of the interface divisions (div_markstein, div_shared) and
+// the per-lane mask of the wave-uniform interface form (D3Q19; D3Q27 runs its interface form under
+// a per-lane branch, where the mask is true).
+template <int Q>
 __device__ __forceinline__ void iface_finish(Iface &f, const Mom &s, const Coeffs &c, int conservative) {
     f.af = sqrtf(f.Fx * f.Fx + f.Fy * f.Fy + f.Fz * f.Fz);
     f.af2 = f.af * f.af;
@@ -278,11 +276,9 @@ __device__ __forceinline__ void iface_finish(Iface &f, const Mom &s, const Coeff
     f.ha = 0.5f * c.Acoef * f.af;
     const float rho = s.rr + s.rb;
     f.coeff = c.beta * (s.rr * s.rb) / (rho * rho);
-    if constexpr (SHARED) {
-        f.rinv_af2 = 1.0f / f.af2;
-        f.rinv_af = 1.0f / f.af;
-        f.omega = f.omega && f.on;
-    }
+    f.rinv_af2 = 1.0f / f.af2;
+    f.rinv_af = 1.0f / f.af;
+    f.omega = f.omega && f.on;
 }
 
 // Colour gradient F = (1/cs2) sum_{j>=1} w_j psi(x + c_j) c_j (forces.cuh:20-60) with psi read
@@ -327,7 +323,7 @@ __device__ __forceinline__ void colour_gradient(PsiAt &&psi_at, float &Fx, float
 // (Omega2 and recolouring, lbm.cuh:187-208) at compile time so the per-pair work has no branches;
 // inside it the |F| <= 1e-4 and conservative cases are selects. Adding the +0 that a skipped
 // Omega2 or recolouring contributes changes no value.
-template <int Q, uint32_t PMASK, bool REST, bool FORCED, bool IFACE, bool SHARED, typename Sink>
+template <int Q, uint32_t PMASK, bool REST, bool FORCED, bool IFACE, typename Sink>
 __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, const Iface &fi, Sink &&sink) {
     constexpr bool forced = FORCED;
     const float rho = s.rr + s.rb;
@@ -386,20 +382,7 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
                 gi = feqi + c.oms * gneqi;
                 go = feqo + c.oms * gneqo;
             }
-            if constexpr (IFACE && !SHARED) {
-                const float Fc = cdot<I>(fi.Fx, fi.Fy, fi.Fz);
-                const float c2q = (Fc * Fc) / fi.af2;
-                const float c2 = fi.small ? 0.0f : c2q;
-                // a skipped Omega2 adds -0, which leaves every value as it is
-                const float om = fi.omega ? fi.ha * (w * c2 - bcoef<Q>(I)) : -0.0f;
-                gi += om;
-                go += om;
-                const float cq = Fc * invcnorm(I) / fi.af;
-                const float cphi = fi.small ? 0.0f : cq;
-                const float D = fi.coeff * cphi * wr;
-                sink(std::integral_constant<int, I>{}, aR * gi + D, aB * gi - D);
-                sink(std::integral_constant<int, O>{}, aR * go - D, aB * go + D);
-            } else if constexpr (IFACE) {
+            if constexpr (IFACE) {
                 const float Fc = cdot<I>(fi.Fx, fi.Fy, fi.Fz);
                 const float c2q = div_markstein(Fc * Fc, fi.af2, fi.rinv_af2);
                 const float c2 = fi.small ? 0.0f : c2q;
@@ -437,7 +420,7 @@ __device__ __forceinline__ void emit_regular(const Mom &s, const Coeffs &c, cons
 // emit_regular hands to the sink. sink(integral_constant<I>, integral_constant<bool, OMASK bit>,
 // fr, fb); the member the lane took is O exactly when that bit differs from `flip`. No body-force
 // form (the forced kernels emit both members).
-template <int Q, uint32_t PMASK, uint32_t OMASK, bool IFACE, bool SHARED, typename Sink>
+template <int Q, uint32_t PMASK, uint32_t OMASK, bool IFACE, typename Sink>
 __device__ __forceinline__ void emit_one(const Mom &s, const Coeffs &c, const Iface &fi, uint32_t flip, Sink &&sink) {
     const float rho = s.rr + s.rb;
     const float inv = 1.0f / (s.rr + s.rb);
@@ -467,16 +450,7 @@ __device__ __forceinline__ void emit_one(const Mom &s, const Coeffs &c, const If
             float g = feq + c.oms * (w * (An2 + An3));
             const auto IC = std::integral_constant<int, I>{};
             const auto OC = std::integral_constant<bool, OM>{};
-            if constexpr (IFACE && !SHARED) {
-                const float Fc = cdot<I>(fi.Fx, fi.Fy, fi.Fz);
-                const float c2q = (Fc * Fc) / fi.af2;
-                const float c2 = fi.small ? 0.0f : c2q;
-                g += fi.omega ? fi.ha * (w * c2 - bcoef<Q>(I)) : -0.0f;
-                const float cq = Fc * invcnorm(I) / fi.af;
-                const float cphi = fi.small ? 0.0f : cq;
-                const float D = neg(fi.coeff * cphi * wr);
-                sink(IC, OC, aR * g + D, aB * g - D);
-            } else if constexpr (IFACE) {
+            if constexpr (IFACE) {
                 const float Fc = cdot<I>(fi.Fx, fi.Fy, fi.Fz);
                 const float c2q = div_markstein(Fc * Fc, fi.af2, fi.rinv_af2);
                 const float c2 = fi.small ? 0.0f : c2q;

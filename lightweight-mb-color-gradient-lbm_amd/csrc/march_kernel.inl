@@ -347,8 +347,6 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
     constexpr int NRING = 64 * RW;
     constexpr int PX = TX + 4, PY = TY + 4, NPSI = PX * PY;
     constexpr int NPASS = (NPSI + NRING - 1) / NRING;  // psi cells per ring lane
-    // interface divisions through shared reciprocals (see iface_finish)
-    constexpr bool kShared = Q == 19 || YBC == kYPeriodic;
     constexpr uint32_t ALL = pair_mask_all<Q>();
     constexpr uint32_t MX = pair_mask_where<Q>(0), MY = pair_mask_where<Q>(1), MZ = pair_mask_where<Q>(2);
 
@@ -583,10 +581,10 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
                     if constexpr (REC) ion = f.on;
                     if (iface) {
                         colour_gradient<Q>(psi_reader(p), f.Fx, f.Fy, f.Fz);
-                        iface_finish<Q, kShared>(f, cur, c, a.conservative);
-                        emit_regular<Q, M, R, FORCED, true, kShared>(cur, c, f, sink);
+                        iface_finish<Q>(f, cur, c, a.conservative);
+                        emit_regular<Q, M, R, FORCED, true>(cur, c, f, sink);
                     } else {
-                        emit_regular<Q, M, R, FORCED, false, kShared>(cur, c, f, sink);
+                        emit_regular<Q, M, R, FORCED, false>(cur, c, f, sink);
                     }
                 }
             };
@@ -617,10 +615,10 @@ __device__ __forceinline__ void march_tile(const MarchArgs &a, Shared<Q, TY, NBU
                     const bool iface = Q == 19 ? (__builtin_amdgcn_ballot_w64(f.on) != 0) : f.on;
                     if (iface) {
                         colour_gradient<Q>(psi_reader(p), f.Fx, f.Fy, f.Fz);
-                        iface_finish<Q, kShared>(f, cur, c, a.conservative);
-                        emit_one<Q, M, OM, true, kShared>(cur, c, f, flip, sink1);
+                        iface_finish<Q>(f, cur, c, a.conservative);
+                        emit_one<Q, M, OM, true>(cur, c, f, flip, sink1);
                     } else {
-                        emit_one<Q, M, OM, false, kShared>(cur, c, f, flip, sink1);
+                        emit_one<Q, M, OM, false>(cur, c, f, flip, sink1);
                     }
                 }
             };
